@@ -1,0 +1,173 @@
+"""Batched dyad reaction-diffusion benchmark (one GPU).
+
+A "step" advances every site of the batch by one dt: channel kernel (mouth
+snapshot, RyR currents, stochastic gating) + the fused state sweep, replayed
+from a captured HIP graph.  Inputs are resident in HBM before timing starts.
+
+Workload: 4096 independent release sites, 81 x 81 x 5 nodes each (320 x 320 x
+16 nm at h = 4 nm), free Ca2+ + ATP-like + calmodulin-like mobile buffers +
+a membrane buffer on the two z-faces, 8..100 RyRs per site in random
+clusters, SheepCannell-form gating, dt = 10 ns, FP64 state (FP32 reported
+beside it).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "simulated dyad-ms/s on 1 GPU (plain vs surrogate) at ≤2% Ca err; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk):
+    sp = workloads.batched_dyads(n_sites=n_sites, precision=precision)
+    dev = DyadHIP(sp)
+    dev.InitOpeningAll()
+    ri, rb = sp.rim_arrays()
+    dev.Update(ri, rb, sp.jsr_array([sp.meta["jsr_uM"]]))
+    dt = sp.meta["dt_ms"]
+    chunk = min(chunk, steps)
+    rem = steps % chunk
+    # warmup also captures the graphs used in the timed region
+    done = 0
+    while done < max(warmup, 1):
+        dev.Step(dt, chunk)
+        done += chunk
+    if rem:
+        dev.Step(dt, rem)
+    dev.Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // chunk):
+        dev.Step(dt, chunk)
+    if rem:
+        dev.Step(dt, rem)
+    dev.Synchronize()
+    t = time.perf_counter() - t0
+    # per-kernel durations, live, with HIP events on the handle's stream
+    ms_sweep, ms_chan = dev.StepProfiled(dt, steps)
+    ca = dev.GetField(0)
+    ok = bool(np.isfinite(ca).all())
+    bps = dev.BytesPerSiteStep()
+    open_frac = float((dev.GetChannelsStates() == 1).sum()) / sum(len(s) for s in sp.sites)
+    res = {
+        "sites": n_sites, "grid": "%dx%dx%d" % (sp.nx, sp.ny, sp.nz), "dt_ms": dt,
+        "wall_s": t, "ms_per_step": 1e3 * t / steps,
+        "dyad_ms_per_s": n_sites * steps * dt / t,
+        "bytes_per_site_step": bps,
+        "sweep_kernel_ms": ms_sweep / steps, "channel_kernel_ms": ms_chan / steps,
+        "sweep_gbs": n_sites * bps / (ms_sweep / steps * 1e-3) / 1e9,
+        "finite": ok, "open_fraction": open_frac,
+        "workload": sp.meta["workload"],
+    }
+    dev.close()
+    return res
+
+
+def cpu_baseline(workloads, n_sites=32, steps=300):
+    """The oracle (C restatement of DyadRDCPU, OpenMP over sites as
+    ParallelHybridSolver does) on the first n_sites sites of the same batch."""
+    from oracle import oracle as O
+    O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    sp = workloads.batched_dyads(n_sites=4096)
+    sp.sites = sp.sites[:n_sites]
+    sp.rng_states = sp.rng_states[:n_sites]
+    sp.init_channel = sp.init_channel[:n_sites]
+    o = O.Oracle(sp)
+    o.init_opening()
+    ri, rb = sp.rim_arrays()
+    o.update(ri, rb, sp.jsr_array([1000.0]))
+    dt = sp.meta["dt_ms"]
+    o.step(dt, 2, threads)
+    t0 = time.perf_counter()
+    o.step(dt, steps, threads)
+    t = time.perf_counter() - t0
+    return {"value": n_sites * steps * dt / t, "unit": "dyad-ms/s", "cores": threads, "kind": "port",
+            "sample": "first %d sites of the same 4096-site batch, %d steps (%.1f s wall)" % (n_sites, steps, t)}
+
+
+def traffic_from_profiles():
+    """HBM bytes per sweep-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, KiB, scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "k_step_fp64_batched4096_pmc.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (ValueError, KeyError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--sites", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: sites are independent and this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from casim_amd import abi, workloads
+    from casim_amd.dyad import DyadHIP
+    abi.load()
+    log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
+    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk)
+    log("fp64:", json.dumps(r64))
+    r32 = None
+    if not a.no_fp32:
+        log("fp32 leg")
+        r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk)
+        log("fp32:", json.dumps(r32))
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(workloads)
+        log("cpu:", json.dumps(cpu))
+    launch_bytes = a.sites * r64["bytes_per_site_step"]
+    achieved = launch_bytes / (r64["sweep_kernel_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": r64["dyad_ms_per_s"],
+        "unit": "dyad-ms/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": r64["workload"], "sites": a.sites, "grid": r64["grid"], "dt_ns": 10,
+                   "path": "plain batched RD, graph of %d steps" % min(a.chunk, a.steps)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles()},
+        "cpu_baseline": cpu,
+        "hbm_gbs_whole_step": a.sites * r64["bytes_per_site_step"] / (r64["ms_per_step"] * 1e-3) / 1e9,
+        "fp32": None if r32 is None else {
+            "value": r32["dyad_ms_per_s"], "ms_per_step": r32["ms_per_step"], "sweep_gbs": r32["sweep_gbs"],
+            "sweep_kernel_ms": r32["sweep_kernel_ms"]},
+        "kernels": {"sweep_ms": r64["sweep_kernel_ms"], "channels_ms": r64["channel_kernel_ms"]},
+        "finite": r64["finite"] and (r32 is None or r32["finite"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,524 @@
+// dyad_kernels.hpp -- gfx950 kernels of the dyad reaction-diffusion step.
+//
+// Device layout (per handle, two ping-pong copies of each):
+//   grid fields  [g][site][x][y][z]   g = ions, then bound fields that exist
+//                                     on every plane (z fastest, as the
+//                                     reference's IndexMP / _Index)
+//   face fields  [f][site][face][x][y] bound fields of membrane-only buffers,
+//                                     face 0 = plane z = 0, face 1 = z = nz-1
+// Free buffer = total - sum of its bound fields is derived where needed, not
+// stored.  Rim nodes are never written by the step kernel, so both copies
+// carry the same rim.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mrg32k3a.hpp"
+
+namespace casim {
+
+constexpr int MAXI = 4;   // ions
+constexpr int MAXB = 8;   // bound fields
+constexpr int MAXU = 8;   // unique buffers
+constexpr int MAXT = 4;   // channel types
+constexpr int NPAR = 16;  // kinetic parameters per type
+constexpr int MAX_SITE_CHANNELS = 127;
+constexpr int JUMP_N = 256;  // jump table entries: A^(2p), p < JUMP_N
+
+struct Params {
+    int dim, nx, ny, nz, S;
+    int NS;    // nodes per site
+    int nFP;   // nodes per face plane (nx*ny)
+    int ni, nb, nu;
+    int n_grid, n_face;
+    int bion[MAXB], bbuf[MAXB], bface[MAXB], bslot[MAXB];  // slot: grid index or face index
+    double Dion[MAXI], Db[MAXB], kon[MAXB], koff[MAXB], binit[MAXB];
+    double total[MAXU];
+    int ufaces[MAXU];
+    int sl[MAXI];
+    double N1[MAXI], K1[MAXI], N2[MAXI], K2[MAXI];
+    double c0[MAXI];
+    double ihx2, ihy2, ihz2;
+};
+
+struct Channels {
+    const int *off;     // [S+1]
+    const int *node;    // [nch] flat node of channel (channel order)
+    const int *type;    // [nch]
+    const int *perm;    // [nch] position of the channel in its site's node-sorted list
+    const int *snode;   // [nch] node-sorted list of mouth nodes per site
+    int *state;         // [nch]
+    uint32_t *rng;      // [S*6]
+    const uint32_t *jump;  // [JUMP_N*18]
+    double *mouths;     // [K*S*M]  K = ni + nb + nu
+    const double *jsr;  // [M*S*ni]
+    double *sJ;         // [ni*nch] currents in node-sorted order
+    double *totals;     // [ni*S]
+    int M;              // max channels per site
+    int nch;            // channels over all sites
+    int ca, mg;
+    int model[MAXT];
+    double par[MAXT * NPAR];
+    double g[MAXT * MAXI];
+};
+
+enum ChannelMode { CH_SNAPSHOT = 1, CH_CURRENTS = 2, CH_GATE = 4, CH_GATE_STORED = 8, CH_ACCUMULATE = 16 };
+
+template <typename T>
+__device__ __forceinline__ T load_face(const T *f, const Params &p, int slot, int s, int face, int xy) {
+    return f[((size_t)slot * p.S + s) * 2 * p.nFP + (size_t)face * p.nFP + xy];
+}
+
+// Total of unique buffer u at plane z: membrane-only buffers vanish off
+// their two planes.
+__device__ __forceinline__ double total_at(const Params &p, int u, int z) {
+    return (p.ufaces[u] && p.dim == 3 && z != 0 && z != p.nz - 1) ? 0.0 : p.total[u];
+}
+
+// ---------------------------------------------------------------------------
+// The step: for every interior node of every site, the 7-point (5-point)
+// Laplacian of each diffusing field, the buffer reactions, the RyR current at
+// mouth nodes, the sarcolemmal rapid-buffer factor, and the explicit Euler
+// update (DyadRDCPU.cpp:404-449 _EvolutionOp, :472-492 _UpdateEvo, :451-470
+// _GetFreeBuffers, with RunRD's Jacobi order :495-506).
+// One thread per node; one block row of blocks per site.
+// NI/NB > 0: compile-time species counts; 0: runtime counts up to MAXI/MAXB.
+template <typename T, int DIM, int NI, int NB>
+__global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__restrict__ gout,
+                                              const T *__restrict__ fin, T *__restrict__ fout,
+                                              const int *__restrict__ ch_off, const int *__restrict__ snode,
+                                              const double *__restrict__ sJ, int nch, Params p, double dt_) {
+    constexpr int LI = NI ? NI : MAXI;
+    constexpr int LB = NB ? NB : MAXB;
+    const int s = blockIdx.y;
+    const int n0 = blockIdx.x * blockDim.x;
+    const int n = n0 + threadIdx.x;
+    const T dt = (T)dt_;
+
+    // Mouth entries of this block: node-sorted list, binary search (uniform).
+    int lo = __builtin_amdgcn_readfirstlane(ch_off[s]);
+    int hi = __builtin_amdgcn_readfirstlane(ch_off[s + 1]);
+    {
+        int a = lo, b = hi;
+        while (a < b) {
+            int m = (a + b) >> 1;
+            if (snode[m] < n0) a = m + 1; else b = m;
+        }
+        int c = a, e = hi;
+        const int n1 = n0 + (int)blockDim.x;
+        while (c < e) {
+            int m = (c + e) >> 1;
+            if (snode[m] < n1) c = m + 1; else e = m;
+        }
+        lo = a;
+        hi = c;
+    }
+
+    if (n >= p.NS) return;
+    const int z = DIM == 3 ? n % p.nz : 0;
+    const int t = DIM == 3 ? n / p.nz : n;
+    const int y = t % p.ny;
+    const int x = t / p.ny;
+    if (x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1) return;
+
+    const size_t fs = (size_t)p.S * p.NS;  // field stride
+    const size_t c = (size_t)s * p.NS + n;
+    const int sx = p.ny * p.nz, sy = p.nz;
+    const int zl = DIM == 3 ? (z == 0 ? 1 : -1) : 0;
+    const int zr = DIM == 3 ? (z == p.nz - 1 ? -1 : 1) : 0;
+    const T ihx2 = (T)p.ihx2, ihy2 = (T)p.ihy2, ihz2 = (T)p.ihz2;
+
+    T C[LI], evoC[LI], B[LB], evoB[LB];
+    const bool face = DIM == 3 && (z == 0 || z == p.nz - 1);
+    const int xy = x * p.ny + y;
+
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        const T *u = gin + j * fs + c;
+        T vc = u[0];
+        C[j] = vc;
+        T lap = (u[sx] - 2 * vc + u[-sx]) * ihx2 + (u[sy] - 2 * vc + u[-sy]) * ihy2;
+        if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+        evoC[j] = (T)p.Dion[j] * lap;
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        if (p.bface[b]) {
+            B[b] = face ? load_face(fin, p, p.bslot[b], s, z == 0 ? 0 : 1, xy) : (T)0;
+            evoB[b] = 0;
+        } else {
+            const T *u = gin + p.bslot[b] * fs + c;
+            T vc = u[0];
+            B[b] = vc;
+            if (p.Db[b] != 0.0) {
+                T lap = (u[sx] - 2 * vc + u[-sx]) * ihx2 + (u[sy] - 2 * vc + u[-sy]) * ihy2;
+                if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+                evoB[b] = (T)p.Db[b] * lap;
+            } else {
+                evoB[b] = 0;
+            }
+        }
+    }
+    // free buffers from the bound fields, subtracted in bound order
+    T fr[MAXU];
+#pragma unroll
+    for (int u = 0; u < MAXU; ++u) {
+        if (u >= p.nu) break;
+        fr[u] = (T)total_at(p, u, z);
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        fr[p.bbuf[b]] -= B[b];
+    }
+    // reactions R1 = koff B - kon C Bfree
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        T R = 0;
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+            if (!NB && b >= p.nb) break;
+            if (p.bion[b] != j) continue;
+            if (p.bface[b] && !face) continue;
+            T R1 = (T)p.koff[b] * B[b] - (T)p.kon[b] * C[j] * fr[p.bbuf[b]];
+            evoB[b] += -R1;
+            R += R1;
+        }
+        evoC[j] += R;
+    }
+    // currents at mouth nodes (the last channel on a node wins, as the
+    // reference's assignment does)
+    for (int k = lo; k < hi; ++k) {
+        if (snode[k] == n) {
+#pragma unroll
+            for (int j = 0; j < LI; ++j) {
+                if (!NI && j >= p.ni) break;
+                evoC[j] = evoC[j] + (T)sJ[(size_t)j * nch + k];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        T e = evoC[j];
+        if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
+            T s1 = (T)p.K1[j] + C[j];
+            T s2 = (T)p.K2[j] + C[j];
+            e *= (T)1 / ((T)1 + (T)p.N1[j] * (T)p.K1[j] / (s1 * s1) + (T)p.N2[j] * (T)p.K2[j] / (s2 * s2));
+        }
+        gout[j * fs + c] = C[j] + e * dt;
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        if (p.bface[b]) {
+            if (face)
+                fout[((size_t)p.bslot[b] * p.S + s) * 2 * p.nFP + (size_t)(z == 0 ? 0 : 1) * p.nFP + xy] =
+                    B[b] + evoB[b] * dt;
+        } else {
+            gout[p.bslot[b] * fs + c] = B[b] + evoB[b] * dt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic rates (SheepCannell.cpp:12-18, TwoStateModel.cpp:4-8,
+// TwoStatePoModel.cpp:3-8, TwoStateMWC.cpp:15-23, TwoStateaHTG.cpp:18-29).
+__device__ __forceinline__ double channel_rate(int model, const double *q, int state, double ca, double mg) {
+    if (model == 1) {
+        if (state) return q[3] * pow(ca * 1e-3, q[4]) * 1e-3;
+        double v = q[0] * pow(ca * 1e-3, q[1]);
+        return ((q[2] < v) ? q[2] : v) * 1e-3;
+    }
+    if (model == 2) {
+        double fCa = q[0], KCa = q[1], fMg = q[2], KMg = q[3], KO0 = q[4], KMgI = q[5], kclose = q[6];
+        if (state) return kclose;
+        double inac = KMgI * KMgI / (KMgI * KMgI + mg * mg);
+        double open = pow(ca + KCa * fCa * (1 + mg / (fMg * KMg)), 4.0);
+        double close = KO0 * pow(fCa * (ca + KCa * (1 + mg / KMg)), 4.0);
+        double P = inac * open / (open + close);
+        return kclose * P / (1 - P);
+    }
+    if (model == 3) {
+        double fCa = q[0], KCa = q[1], fMg = q[2], KMg = q[3], KO0 = q[4], KMgI = q[5];
+        double KCL = q[6], KI = q[7], KOL = q[8], kclose = q[9];
+        if (state) return kclose;
+        double mult = KMgI * KMgI / (KMgI * KMgI + mg * mg);
+        double up = (KCL * KI * (pow(ca, 4.0) * pow(fMg, 4.0) * pow(fCa, 4.0) * pow(KMg, 4.0) * KO0 +
+                                 pow(ca * fMg * KMg + fCa * (mg + fMg * KMg) * KCa, 4.0) * KOL));
+        double down =
+            (4 * pow(ca, 3.0) * pow(fMg, 3.0) * fCa * pow(KMg, 3.0) * KCa * KCL * KI *
+                 (mg + fMg * KMg + fMg * pow(fCa, 3.0) * (mg + KMg) * KO0) * KOL +
+             6 * pow(ca, 2.0) * pow(fMg, 2.0) * pow(fCa, 2.0) * pow(KMg, 2.0) * pow(KCa, 2.0) * KCL * KI *
+                 (pow(mg + fMg * KMg, 2.0) + pow(fMg, 2.0) * pow(fCa, 2.0) * pow(mg + KMg, 2.0) * KO0) * KOL +
+             4 * ca * fMg * pow(fCa, 3.0) * KMg * pow(KCa, 3.0) * KCL * KI *
+                 (pow(mg + fMg * KMg, 3.0) + pow(fMg, 3.0) * fCa * pow(mg + KMg, 3.0) * KO0) * KOL +
+             pow(fCa, 4.0) * pow(KCa, 4.0) * KCL * KI *
+                 (pow(mg + fMg * KMg, 4.0) + pow(fMg, 4.0) * pow(mg + KMg, 4.0) * KO0) * KOL +
+             pow(ca, 4.0) * pow(fMg, 4.0) * pow(KMg, 4.0) *
+                 (KCL * KI * KOL + pow(fCa, 4.0) * KO0 * (1 + KI + KCL * KI + KCL * KI * KOL)));
+        double P = mult * up / down;
+        return kclose * P / (1 - P);
+    }
+    return 0.0;
+}
+
+// Draw number p (0-based) after state S: jump 2p steps, then one U01d draw.
+__device__ __forceinline__ double draw_at(const uint32_t *__restrict__ jump, const uint32_t S[6], int p) {
+    const uint32_t *J = jump + (size_t)p * 18;
+    uint32_t A1[9], A2[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        A1[i] = J[i];
+        A2[i] = J[9 + i];
+    }
+    uint32_t st[6];
+    mrg::matvec(A1, S, st, mrg::C1, mrg::M1);
+    mrg::matvec(A2, S + 3, st + 3, mrg::C2, mrg::M2);
+    return mrg::draw(st);
+}
+
+// ---------------------------------------------------------------------------
+// Channel kernel, one 64-lane wave per site, lanes over channels.
+//  CH_SNAPSHOT  mouth snapshot of ions, bound and free buffers
+//               (DyadRDCPU.cpp:371-390) from the current grid
+//  CH_CURRENTS  J = state * g (jSR - mouth) per ion (PassiveIonChannel.cpp:28-34,
+//               DyadRDCPU.cpp:250-271) into the node-sorted current list, and
+//               the site total (ion 0 restarts from zero, other ions keep
+//               accumulating, as DyadRDCPU.cpp:254,260 do) when CH_ACCUMULATE
+//  CH_GATE      DyadChannels::RunMC (DyadChannels.cpp:97-130) on the fresh
+//               snapshot; CH_GATE_STORED on the stored one.
+// The stream is consumed in channel order: one draw per channel, one more
+// when it toggles.  Lanes evaluate their draw speculatively at position
+// (draws so far + lanes before them) by jumping the step-start state with
+// A^(2p); a ballot finds the first toggling channel, positions after it are
+// re-evaluated with the extra draw accounted for, until a ballot is empty.
+template <typename T>
+__global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const T *__restrict__ f, Channels a,
+                                                 Params p, int mode, double dt) {
+    __shared__ double Jl[MAXI][64];
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int off = a.off[s];
+    const int n = a.off[s + 1] - off;
+    const size_t fs = (size_t)p.S * p.NS;
+    uint32_t S6[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
+    int pos = 0;
+    double acc[MAXI];
+#pragma unroll
+    for (int k = 0; k < MAXI; ++k)
+        acc[k] = (k == 0 || !(mode & CH_ACCUMULATE)) ? 0.0 : (k < p.ni ? a.totals[s + k * p.S] : 0.0);
+
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int i = c0 + lane;
+        const bool valid = i < n;
+        int st = valid ? a.state[off + i] : 0;
+        const int typ = valid ? a.type[off + i] : 0;
+        double ion[MAXI], bnd[MAXB], fre[MAXU];
+        if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE)) {
+            const int node = valid ? a.node[off + i] : 0;
+            const int z = p.dim == 3 ? node % p.nz : 0;
+            const int xy = p.dim == 3 ? node / p.nz : node;
+            const size_t c = (size_t)s * p.NS + node;
+#pragma unroll
+            for (int j = 0; j < MAXI; ++j) ion[j] = (j < p.ni) ? (double)g[j * fs + c] : 0.0;
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) {
+                if (b >= p.nb) {
+                    bnd[b] = 0.0;
+                } else if (p.bface[b]) {
+                    bool fc = p.dim == 3 && (z == 0 || z == p.nz - 1);
+                    bnd[b] = fc ? (double)load_face(f, p, p.bslot[b], s, z == 0 ? 0 : 1, xy) : 0.0;
+                } else {
+                    bnd[b] = (double)g[p.bslot[b] * fs + c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < MAXU; ++u) fre[u] = u < p.nu ? total_at(p, u, z) : 0.0;
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b)
+                if (b < p.nb) fre[p.bbuf[b]] -= bnd[b];
+            if ((mode & CH_SNAPSHOT) && valid) {
+                for (int k = 0; k < p.ni; ++k) a.mouths[i + (size_t)a.M * (s + (size_t)k * p.S)] = ion[k];
+                for (int k = 0; k < p.nb; ++k)
+                    a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni) * p.S)] = bnd[k];
+                for (int k = 0; k < p.nu; ++k)
+                    a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni + p.nb) * p.S)] = fre[k];
+            }
+        }
+        if (mode & CH_CURRENTS) {
+            const int sp = valid ? a.perm[off + i] : 0;
+#pragma unroll
+            for (int k = 0; k < MAXI; ++k) {
+                double J = 0.0;
+                if (k < p.ni && valid) {
+                    double cond = a.g[typ * MAXI + k] * (a.jsr[i + (size_t)a.M * (s + (size_t)k * p.S)] - ion[k]);
+                    J = (double)st * cond;
+                    a.sJ[(size_t)k * a.nch + off + sp] = J;
+                }
+                Jl[k][lane] = J;
+            }
+            __syncthreads();
+            if (mode & CH_ACCUMULATE) {
+                const int m = (n - c0) < 64 ? (n - c0) : 64;
+#pragma unroll
+                for (int k = 0; k < MAXI; ++k)
+                    for (int q = 0; q < m; ++q) acc[k] += Jl[k][q];
+            }
+            __syncthreads();
+        }
+        if (mode & (CH_GATE | CH_GATE_STORED)) {
+            double ca, mg;
+            if (mode & CH_GATE_STORED) {
+                ca = (a.ca >= 0 && valid) ? a.mouths[i + (size_t)a.M * (s + (size_t)a.ca * p.S)] : 0.0;
+                mg = (a.mg >= 0 && valid) ? a.mouths[i + (size_t)a.M * (s + (size_t)a.mg * p.S)] : 0.0;
+            } else {
+                ca = a.ca >= 0 ? ion[a.ca] : 0.0;
+                mg = a.mg >= 0 ? ion[a.mg] : 0.0;
+            }
+            const double thr = valid ? channel_rate(a.model[typ], a.par + typ * NPAR, st, ca, mg) * dt : -1.0;
+            const int end = (n - c0) < 64 ? n : c0 + 64;
+            int first = c0;
+            while (first < end) {
+                const bool active = valid && i >= first;
+                double u = active ? draw_at(a.jump, S6, pos + (i - first)) : 2.0;
+                const bool tr = active && (u < thr);
+                const unsigned long long m = __ballot(tr);
+                if (m == 0ull) {
+                    pos += end - first;
+                    break;
+                }
+                const int jl = __builtin_ctzll(m);
+                if (lane == jl) st ^= 1;
+                pos += (c0 + jl - first) + 2;
+                first = c0 + jl + 1;
+            }
+            if (valid) a.state[off + i] = st;
+        }
+    }
+    if ((mode & CH_CURRENTS) && (mode & CH_ACCUMULATE) && lane == 0) {
+        for (int k = 0; k < p.ni; ++k) a.totals[s + k * p.S] = acc[k];
+    }
+    if ((mode & (CH_GATE | CH_GATE_STORED)) && lane == 0) {
+        const uint32_t *J = a.jump + (size_t)pos * 18;
+        uint32_t out[6];
+        mrg::matvec(J, S6, out, mrg::C1, mrg::M1);
+        mrg::matvec(J + 9, S6 + 3, out + 3, mrg::C2, mrg::M2);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a.rng[s * 6 + q] = out[q];
+    }
+}
+
+// DyadChannels::InitOpening (DyadChannels.cpp:83-95): n+1 draws, all closed,
+// then `channel` open.  One thread per site.
+__global__ void k_init_opening(Channels a, int S, int site, int channel, const int *__restrict__ chans) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || (site >= 0 && s != site)) return;
+    const int off = a.off[s], n = a.off[s + 1] - off;
+    const int ch = chans ? chans[s] : channel;
+    for (int k = 0; k < n; ++k) a.state[off + k] = (k == ch) ? 1 : 0;
+    uint32_t S6[6], out[6];
+    for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
+    const uint32_t *J = a.jump + (size_t)(n + 1) * 18;
+    mrg::matvec(J, S6, out, mrg::C1, mrg::M1);
+    mrg::matvec(J + 9, S6 + 3, out + 3, mrg::C2, mrg::M2);
+    for (int i = 0; i < 6; ++i) a.rng[s * 6 + i] = out[i];
+}
+
+// Reset (DyadRDCPU.cpp:203-229) of both ping-pong copies.
+template <typename T>
+__global__ void k_reset(T *g0, T *g1, T *f0, T *f1, Params p) {
+    const size_t fs = (size_t)p.S * p.NS;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < fs) {
+        for (int j = 0; j < p.ni; ++j) g0[j * fs + idx] = g1[j * fs + idx] = (T)p.c0[j];
+        for (int b = 0; b < p.nb; ++b)
+            if (!p.bface[b]) g0[p.bslot[b] * fs + idx] = g1[p.bslot[b] * fs + idx] = (T)p.binit[b];
+    }
+    const size_t ff = (size_t)p.S * 2 * p.nFP;
+    if (idx < ff)
+        for (int b = 0; b < p.nb; ++b)
+            if (p.bface[b]) f0[p.bslot[b] * ff + idx] = f1[p.bslot[b] * ff + idx] = (T)p.binit[b];
+}
+
+// _SetBoundaries (DyadRDCPU.cpp:290-320) on both copies.
+template <typename T>
+__global__ void k_rim(T *g0, T *g1, T *f0, T *f1, const double *__restrict__ rim_ions,
+                      const double *__restrict__ rim_bound, Params p) {
+    const int s = blockIdx.y;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.NS) return;
+    const int z = p.dim == 3 ? n % p.nz : 0;
+    const int t = p.dim == 3 ? n / p.nz : n;
+    const int y = t % p.ny, x = t / p.ny;
+    if (!(x == 0 || x == p.nx - 1 || y == 0 || y == p.ny - 1)) return;
+    const size_t fs = (size_t)p.S * p.NS;
+    const size_t c = (size_t)s * p.NS + n;
+    for (int j = 0; j < p.ni; ++j) g0[j * fs + c] = g1[j * fs + c] = (T)rim_ions[s + j * p.S];
+    for (int b = 0; b < p.nb; ++b) {
+        const T v = (T)rim_bound[s + b * p.S];
+        if (!p.bface[b]) {
+            g0[p.bslot[b] * fs + c] = g1[p.bslot[b] * fs + c] = v;
+        } else if (p.dim == 3 && (z == 0 || z == p.nz - 1)) {
+            const size_t q = ((size_t)p.bslot[b] * p.S + s) * 2 * p.nFP + (size_t)(z == 0 ? 0 : 1) * p.nFP + t;
+            f0[q] = f1[q] = v;
+        }
+    }
+}
+
+// _GetGradients (DyadRDCPU.cpp:322-369, DyadRD2DCPU.cpp:305-350): one thread
+// per (site, field), terms summed in the reference's loop order.
+template <typename T>
+__global__ void k_efflux(const T *__restrict__ g, double *__restrict__ out, Params p, double gm, double sxm,
+                         double sym) {
+#pragma clang fp contract(off)
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = p.ni + p.nb;
+    if (tid >= p.S * nf) return;
+    const int s = tid / nf, fi = tid % nf;
+    double D;
+    const T *u;
+    const size_t fs = (size_t)p.S * p.NS;
+    if (fi < p.ni) {
+        D = p.Dion[fi];
+        u = g + fi * fs + (size_t)s * p.NS;
+    } else {
+        const int b = fi - p.ni;
+        D = p.Db[b];
+        if (p.bface[b]) {
+            out[fi + nf * s] = 0.0;  // membrane buffers do not diffuse
+            return;
+        }
+        u = g + p.bslot[b] * fs + (size_t)s * p.NS;
+    }
+    const int zmax = p.dim == 3 ? p.nz - 1 : 1;
+    const int nz = p.nz, ny = p.ny, nx = p.nx;
+    double acc = 0.0;
+    const double tx = D * gm * sxm, ty = D * gm * sym;
+    for (int x = 0; x < nx; x += nx - 1)
+        for (int y = 1; y < ny - 1; ++y)
+            for (int z = 0; z < zmax; ++z) {
+                int d = x == 0 ? 1 : -1;
+                double u1 = (double)u[z + nz * (y + ny * (x + d))];
+                double u0 = (double)u[z + nz * (y + ny * x)];
+                double u2 = (double)u[z + nz * (y + ny * (x + 2 * d))];
+                acc += tx * (2 * u1 - 1.5 * u0 - 0.5 * u2);
+            }
+    for (int y = 0; y < ny; y += ny - 1)
+        for (int x = 0; x < nx; ++x)
+            for (int z = 0; z < zmax; ++z) {
+                int d = y == 0 ? 1 : -1;
+                double u1 = (double)u[z + nz * (y + d + ny * x)];
+                double u0 = (double)u[z + nz * (y + ny * x)];
+                double u2 = (double)u[z + nz * (y + 2 * d + ny * x)];
+                acc += ty * (2 * u1 - 1.5 * u0 - 0.5 * u2);
+            }
+    out[fi + nf * s] = acc;
+}
+
+}  // namespace casim

@@ -1,0 +1,249 @@
+/*
+ * casim_dyad.h -- C ABI of the MI355X dyadic-cleft reaction-diffusion library
+ * (libcasim_dyad.so).
+ *
+ * The library replaces the dyad compartment of CaSim: the `DyadSeries`
+ * implementations DyadRDCPU / DyadRD2DCPU (CPU) and DyadRDMP / DyadRDMP2D
+ * (CUDA), together with the per-site stochastic RyR gating they drive through
+ * DyadChannels -> KineticModel -> ParallelRng.  Every entry point below names
+ * the reference method it stands in for (file:line in IaBogdi/CaSim, directory
+ * CaSim/).
+ *
+ * Conventions
+ *  - Units are the reference's: nm, ms, uM; D in nm^2/ms; kon in 1/(uM ms);
+ *    koff in 1/ms; channel conductance g in (uM/ms)/uM.
+ *  - "site" = one release unit (one cleft + its RyR cluster); the reference
+ *    calls it a thread / batch slot (idx_batch).
+ *  - Species order is the reference's: ions and buffers in sorted JSON-key
+ *    order, bound fields ion-major over (ion, buffer) pairs
+ *    (DyadRDCPU.cpp:84-99).
+ *  - Host arrays use the reference layouts, noted per argument.  Grid fields
+ *    are [site][x][y][z] with z fastest (DyadRDCPU.cpp:277-279).
+ *  - Every function returns 0 on success or a negative CASIM_E* code; the
+ *    message of the last failure on the calling thread is casim_last_error().
+ *    (The reference has no error reporting; this is the ABI's own.)
+ *  - Threading: one handle must not be used from two threads at once.  All
+ *    device work of a handle is issued on the handle's own HIP stream.
+ */
+#ifndef CASIM_DYAD_H
+#define CASIM_DYAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CASIM_OK 0
+#define CASIM_E_INVALID (-1)   /* bad argument or configuration */
+#define CASIM_E_DEVICE (-2)    /* HIP runtime error */
+#define CASIM_E_STATE (-3)     /* call out of protocol order */
+#define CASIM_E_RANGE (-4)     /* index out of range */
+
+/* Kinetic models of a passive RyR (PassiveIonChannel.cpp:10-24). */
+#define CASIM_MODEL_SHEEP_CANNELL 1 /* SheepCannell.cpp:4-18; params: kopen0, ncaopen, kopenmax, kclose0, ncaclose */
+#define CASIM_MODEL_TWO_STATE_MWC 2 /* TwoStateMWC.cpp:4-23; params: fCa, KCa, fMg, KMg, KO0, KMgI, kclose */
+#define CASIM_MODEL_TWO_STATE_AHTG 3 /* TwoStateaHTG.cpp:4-29; params: fCa, KCa, fMg, KMg, KO0, KMgI, KCL, KI, KOL, kclose */
+#define CASIM_MAX_MODEL_PARAMS 16
+
+/* Arithmetic of the state grid. */
+#define CASIM_FP64 0
+#define CASIM_FP32 1
+
+/*
+ * One batch of sites.  Mirrors what the DyadRDCPU / DyadRD2DCPU constructors
+ * read from the "Dyad" and "jSR" JSON objects (DyadRDCPU.cpp:8-161,
+ * DyadChannels.cpp:17-61), flattened.  All pointers are read during
+ * casim_dyad_create only.
+ */
+typedef struct CasimDyadConfig {
+    int32_t dim;                 /* 3 = DyadRDCPU geometry, 2 = DyadRD2DCPU geometry */
+    int32_t nx, ny, nz;          /* nodes: x/dx + 1 etc. (DyadRDCPU.cpp:18-20); nz = 1 when dim == 2 */
+    double dx, dy, dz;           /* node spacing, nm (dz unused when dim == 2) */
+    double z;                    /* JSON "z": cleft height, nm (2D: element thickness) */
+    double radius;               /* JSON "Radius", nm (efflux scale, DyadRDCPU.cpp:146) */
+    double voltage;              /* JSON "Voltage" */
+    int32_t n_sites;
+
+    int32_t n_ions;              /* sorted JSON keys of "Ions" */
+    int32_t n_buffers;           /* sorted JSON keys of "Buffers" (unique buffers) */
+    int32_t n_bound;             /* (ion, buffer) bound fields, ion-major */
+    const double *ion_D;         /* [n_ions] */
+    const double *ion_c0;        /* [n_ions] resting concentration ("Concentration") */
+    const double *buffer_total;  /* [n_buffers] "Total Concentration" */
+    const int32_t *buffer_faces_only; /* [n_buffers] or NULL.  Extension (no reference
+                                    counterpart): 1 = the buffer exists only on the
+                                    z = 0 and z = nz-1 planes and must have D = 0 */
+    const int32_t *bound_ion;    /* [n_bound] ion index of the bound field */
+    const int32_t *bound_buffer; /* [n_bound] buffer index of the bound field */
+    const double *bound_D;       /* [n_bound] */
+    const double *bound_kon;     /* [n_bound] */
+    const double *bound_koff;    /* [n_bound] */
+    int32_t start_equilibrium;   /* JSON "Start" == "Equilibrium" (DyadRDCPU.cpp:60-81) */
+    const int32_t *sl_binds;     /* [n_ions] 1 if "Sarcolemma" lists the ion */
+    const double *sl_N1, *sl_K1, *sl_N2, *sl_K2; /* [n_ions] */
+    int32_t calcium_index;       /* ion index of "Calcium", -1 if absent */
+    int32_t magnesium_index;     /* ion index of "Magnesium", -1 if absent */
+
+    /* Channel types: the sorted keys of "Channels" (DyadChannels.cpp:30-58). */
+    int32_t n_channel_types;
+    const int32_t *type_model;        /* [n_channel_types] CASIM_MODEL_* */
+    const double *type_params;        /* [n_channel_types * CASIM_MAX_MODEL_PARAMS] */
+    const double *type_conductance;   /* [n_channel_types * n_ions]; g[k] in sorted key
+                                         order of "Conductance" (DyadChannels.cpp:40-43) */
+
+    /* Channels of every site, CSR over sites.  The reference gives every site
+       the same list (DyadChannels.cpp:45-58); per-site lists are an extension
+       for heterogeneous batches. */
+    const int32_t *site_channel_offset; /* [n_sites + 1] */
+    const double *channel_x;            /* [total channels] nm */
+    const double *channel_y;            /* [total channels] nm */
+    const int32_t *channel_type;        /* [total channels] */
+
+    /* MRG32k3a state (Cg[0..5] of RngStream, rngstream.cpp:57) that each
+       site's stream starts from; see casim_rng_stream_states. */
+    const uint64_t *rng_state;          /* [n_sites * 6] */
+
+    int32_t precision;           /* CASIM_FP64 or CASIM_FP32 state grid */
+} CasimDyadConfig;
+
+typedef struct CasimDyad CasimDyad;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *casim_last_error(void);
+
+/* Library build information: "<version> gfx950". */
+const char *casim_version(void);
+
+/*
+ * Initial MRG32k3a states of n consecutive streams after
+ * RngStream::SetPackageSeed(seed): stream k starts at A^(k * 2^127) seed,
+ * exactly as the k-th RngStream constructed after the seed is set
+ * (rngstream.cpp:303-321 constructor, :357-364 SetPackageSeed; seed check
+ * :200-234).  Host-only; no device needed.  out: [n * 6].
+ */
+int casim_rng_stream_states(const uint64_t seed[6], int32_t first_stream, int32_t n, uint64_t *out);
+
+/* Advance one MRG32k3a state by n_draws RandU01() draws (2 generator steps
+ * per draw, rngstream.cpp:271-283 with incPrec = true, :306).  Host-only. */
+int casim_rng_advance(uint64_t state[6], int64_t n_draws);
+
+/* n_draws RandU01() values of one stream, advancing state (rngstream.cpp:485-491). */
+int casim_rng_draws(uint64_t state[6], int32_t n_draws, double *out);
+
+/*
+ * Constructor: DyadRDCPU::DyadRDCPU (DyadRDCPU.cpp:8-161) /
+ * DyadRD2DCPU::DyadRD2DCPU.  Computes the equilibrium bound concentrations
+ * (GaussElimination, DyadRDCPU.cpp:163-193), maps channel coordinates to nodes
+ * (Point::GetCoordsOnGrid, Point.cpp:17-23), allocates the site batch on the
+ * current HIP device and resets it.
+ */
+int casim_dyad_create(const CasimDyadConfig *cfg, CasimDyad **out);
+int casim_dyad_destroy(CasimDyad *d);
+
+/* DyadSeries::Reset -> DyadRDCPU::Reset (DyadRDCPU.cpp:203-229).  All nodes
+ * to the resting/equilibrium state, currents to zero, every channel closed,
+ * every site's RNG back to cfg->rng_state. */
+int casim_dyad_reset(CasimDyad *d);
+
+/* DyadSeries::InitOpening -> DyadChannels::InitOpening (DyadChannels.cpp:83-95):
+ * one draw per channel for SetMacroState(0), then one draw to open `channel`.
+ * site = -1 applies to every site with channel[site] from `channels`
+ * (NULL: the same `channel` everywhere). */
+int casim_dyad_init_opening(CasimDyad *d, int32_t site, int32_t channel, const int32_t *channels);
+
+/*
+ * DyadSeries::Update -> DyadRDCPU::Update (DyadRDCPU.cpp:392-402): snapshot
+ * the ions/buffers at channel mouths (:371-390), write the Dirichlet rim from
+ * the cytosol (:290-320), compute the rim effluxes (:322-369) and the channel
+ * currents (:250-271, PassiveIonChannel::Flux).
+ *   rim_ions   [site + ion * S]
+ *   rim_bound  [site + bound * S]
+ *   jsr        [ch + n_ch * (site + ion * S)]   (n_ch = casim_dyad_max_channels)
+ *   voltage    the cytosol's V
+ */
+int casim_dyad_update(CasimDyad *d, const double *rim_ions, const double *rim_bound,
+                      const double *jsr, double voltage);
+
+/* DyadSeries::RunRD -> DyadRDCPU::RunRD (DyadRDCPU.cpp:495-506) for every
+ * site at once (the reference GPU form, DyadRDMP.cu:853-860).  site must be -1. */
+int casim_dyad_run_rd(CasimDyad *d, double dt, int32_t site);
+
+/* DyadSeries::RunMC -> DyadChannels::RunMC (DyadChannels.cpp:97-130) for
+ * every site at once, on the mouth snapshot of the last update.  site must be -1. */
+int casim_dyad_run_mc(CasimDyad *d, double dt, int32_t site);
+
+/*
+ * Fused path (no reference counterpart as a single call): k_steps repetitions
+ * of the step the reference's batch loop performs with a constant rim and a
+ * constant jSR (ParallelHybridSolver.cpp:408-437 with DyadRDCPU, per step
+ * RunRD + RunMC for every site, then Update).  Equivalent to k_steps times
+ * { run_rd(dt); run_mc(dt); update(rim, jsr) } with the rim and jSR of the
+ * last casim_dyad_update call.  The k_steps launches are captured once into a
+ * HIP graph per (k_steps, dt) and replayed.
+ */
+int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps);
+
+/* The same k_steps as casim_dyad_step, launched eagerly (no graph) with a
+ * HIP event pair around every kernel on the handle's stream.  kernel_ms[0]
+ * receives the summed duration of the state-sweep kernels, kernel_ms[1] that
+ * of the channel kernels.  For measurement; results equal casim_dyad_step's. */
+int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double kernel_ms[2]);
+
+/* Block until the handle's stream is idle. */
+int casim_dyad_synchronize(CasimDyad *d);
+
+/* DyadSeries::GetEffluxes (DyadRDCPU.cpp:273-275): [f + (n_ions + n_bound) * site]. */
+int casim_dyad_get_effluxes(CasimDyad *d, double *out);
+
+/* DyadSeries::GetTotalSRCurrent (DyadRDCPU.cpp:242-244): [site + ion * S]. */
+int casim_dyad_get_total_sr_current(CasimDyad *d, double *out);
+
+/* One field of DyadSeries::GetConcentrations (DyadRDCPU.cpp:516-539):
+ * field < n_ions: ion; else bound field (field - n_ions).  out: [site][x][y][z].
+ * Face-only bound fields read 0 away from their two planes. */
+int casim_dyad_get_field(CasimDyad *d, int32_t field, double *out);
+
+/* Overwrite one field (same layout as casim_dyad_get_field); for tests and restarts. */
+int casim_dyad_set_field(CasimDyad *d, int32_t field, const double *in);
+
+/* DyadSeries::GetChannelsStates (DyadRDCPU.cpp:541-550): [site * n_ch + ch],
+ * n_ch = casim_dyad_max_channels; padding entries read -1. */
+int casim_dyad_get_channel_states(CasimDyad *d, int32_t *out);
+int casim_dyad_set_channel_states(CasimDyad *d, const int32_t *in);
+
+/* Mouth snapshot of the last update, DyadRDCPU::ions_near_channels
+ * (DyadRDCPU.cpp:371-390): [ch + n_ch * (site + k * S)], k over ions, bound
+ * fields, free buffers. */
+int casim_dyad_get_mouths(CasimDyad *d, double *out);
+
+/* RNG state of each site's stream (RngStream::GetState, rngstream.cpp:415-419): [site * 6]. */
+int casim_dyad_get_rng_states(CasimDyad *d, uint64_t *out);
+int casim_dyad_set_rng_states(CasimDyad *d, const uint64_t *in);
+
+/* DyadSeries::GetElementVolume (DyadRDCPU.cpp:195-197, DyadRD2DCPU.cpp:189-191). */
+double casim_dyad_element_volume(const CasimDyad *d);
+/* Largest channel count over the batch's sites (DyadSeries::GetNumSRChannels
+ * for a uniform batch, DyadRDCPU.cpp:199-201). */
+int32_t casim_dyad_max_channels(const CasimDyad *d);
+/* Channel count of one site. */
+int32_t casim_dyad_site_channels(const CasimDyad *d, int32_t site);
+/* Grid node (flat index x*ny*nz + y*nz + z) of one channel's mouth. */
+int32_t casim_dyad_channel_node(const CasimDyad *d, int32_t site, int32_t ch);
+
+/* Algorithmic HBM bytes of one site-step of the state sweep (read every
+ * stored node once, write every updated node once). */
+double casim_dyad_bytes_per_site_step(const CasimDyad *d);
+
+/* Tuning of the state-sweep kernel: planes per workgroup along x (0 = auto)
+ * and threads per workgroup (multiple of 64; 0 = auto). */
+int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block);
+
+/* HIP stream of the handle (hipStream_t), for callers that time with events. */
+void *casim_dyad_stream(CasimDyad *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASIM_DYAD_H */

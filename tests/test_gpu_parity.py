@@ -1,0 +1,182 @@
+"""HIP path (through the C ABI) against the oracle on the same seeded inputs.
+
+Bars: RNG states and channel states bit-exact; every grid node, mouth value,
+site current and efflux within 1e-9 relative in FP64 and 1e-4 relative in
+FP32 (channels clamped for FP32, where rounding of the mouth Ca could
+legitimately flip a stochastic decision)."""
+import os
+
+import numpy as np
+import pytest
+
+from casim_amd import abi, config, workloads
+from casim_amd.dyad import DyadHIP
+from tests.conftest import GOLDEN
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+FP32_TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def setup_pair(oracle_mod, sp, jsr=1000.0):
+    ora = oracle_mod.Oracle(sp)
+    dev = DyadHIP(sp)
+    ora.init_opening()
+    dev.InitOpeningAll()
+    ri, rb = sp.rim_arrays()
+    j = sp.jsr_array([jsr] + [0.0] * (sp.n_ions - 1))
+    ora.update(ri, rb, j)
+    dev.Update(ri, rb, j)
+    return ora, dev, (ri, rb, j)
+
+
+def compare(ora, dev, sp, tol, fields=True):
+    nf = sp.n_ions + sp.n_bound
+    worst = 0.0
+    if fields:
+        for f in range(nf):
+            e = rel_err(dev.GetField(f), ora.field(f))
+            worst = max(worst, e)
+            assert e < tol, ("field", sp.field_names()[f], e)
+    m_dev, m_ora = dev.GetIonsNearChannels(), ora.mouths()
+    st = ora.channel_states()
+    mask = np.broadcast_to(st >= 0, m_ora.shape)
+    assert rel_err(m_dev[mask], m_ora[mask]) < tol
+    assert rel_err(dev.GetTotalSRCurrent(), ora.total_sr_current()) < tol
+    eo, ed = ora.effluxes(), dev.GetEffluxes()
+    scale = np.maximum(np.abs(eo), 1e-6 * np.abs(eo).max(axis=0, keepdims=True) + 1e-300)
+    assert float(np.max(np.abs(ed - eo) / scale)) < max(tol, 1e-8)
+    return worst
+
+
+def test_init_opening_rng_bit_exact(gpu, oracle_mod):
+    sp = workloads.single_dyad(8)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    assert np.array_equal(dev.GetRngStates(), ora.rng_states())
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+    assert dev.GetChannelsStates()[:, 12].sum() == 8
+
+
+@pytest.mark.parametrize("precision,tol", [(abi.CASIM_FP64, FP64_TOL), (abi.CASIM_FP32, FP32_TOL)])
+def test_clamped_single_dyad(gpu, oracle_mod, precision, tol):
+    say("clamped single dyad, precision", precision)
+    sp = workloads.single_dyad(4, clamped=True, precision=precision)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt, n = 1e-5, 1500
+    dev.Step(dt, n)
+    ora.step(dt, n)
+    worst = compare(ora, dev, sp, tol)
+    say("worst rel err", worst)
+    assert np.array_equal(dev.GetRngStates(), ora.rng_states())
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+
+
+def test_stochastic_trajectories_fp64(gpu, oracle_mod):
+    """Normal gating: the same channel histories and RNG states, step by step."""
+    sp = workloads.single_dyad(8, clamped=False)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt = 1e-5
+    for block in range(6):
+        dev.Step(dt, 250)
+        ora.step(dt, 250)
+        assert np.array_equal(dev.GetChannelsStates(), ora.channel_states()), block
+        assert np.array_equal(dev.GetRngStates(), ora.rng_states()), block
+    compare(ora, dev, sp, FP64_TOL)
+    assert ora.channel_states().sum() > 8  # openings spread beyond the initial channel
+
+
+def test_protocol_path_equals_fused_path(gpu, oracle_mod):
+    """Update / RunRD / RunMC one call at a time == the graph-captured fused step."""
+    sp = workloads.single_dyad(3, clamped=False)
+    _, a, (ri, rb, j) = setup_pair(oracle_mod, sp)
+    b = DyadHIP(sp)
+    b.InitOpeningAll()
+    b.Update(ri, rb, j)
+    dt = 1e-5
+    for _ in range(120):
+        a.RunRD(dt)
+        a.RunMC(dt)
+        a.Update(ri, rb, j)
+    b.Step(dt, 120)
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(a.GetField(f), b.GetField(f))
+    assert np.array_equal(a.GetChannelsStates(), b.GetChannelsStates())
+    assert np.array_equal(a.GetRngStates(), b.GetRngStates())
+    assert np.array_equal(a.GetIonsNearChannels(), b.GetIonsNearChannels())
+    assert np.array_equal(a.GetTotalSRCurrent(), b.GetTotalSRCurrent())
+    assert np.array_equal(a.GetEffluxes(), b.GetEffluxes())
+
+
+def test_graph_replays_compose(gpu, oracle_mod):
+    sp = workloads.single_dyad(2, clamped=False)
+    _, a, (ri, rb, j) = setup_pair(oracle_mod, sp)
+    b = DyadHIP(sp)
+    b.InitOpeningAll()
+    b.Update(ri, rb, j)
+    for _ in range(4):
+        a.Step(1e-5, 25)
+    b.Step(1e-5, 100)
+    assert np.array_equal(a.GetField(0), b.GetField(0))
+    assert np.array_equal(a.GetRngStates(), b.GetRngStates())
+
+
+def test_heterogeneous_batch(gpu, oracle_mod):
+    """Sites with 8..100 RyRs in random clusters, one batch, stochastic."""
+    sp = workloads.batched_dyads(n_sites=6, geometry_seed=7)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt = 1e-5
+    dev.Step(dt, 400)
+    ora.step(dt, 400)
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+    assert np.array_equal(dev.GetRngStates(), ora.rng_states())
+    compare(ora, dev, sp, FP64_TOL)
+
+
+def test_reference_test_input_2d(gpu, oracle_mod):
+    """The reference's own test configuration (CPU2D, Ca + Mg, ATP binding
+    both, calmodulin, Fluo-4, sarcolemmal buffer, 20 RyRs, SheepCannell)."""
+    sp, j = config.load_json(os.path.join(GOLDEN, "test_input_Parameters.json"), n_sites=3)
+    sp.init_channel = [0, 5, 11]
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt = float(j["dt, ns"]) * 1e-6
+    dev.Step(dt, 300)
+    ora.step(dt, 300)
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+    assert np.array_equal(dev.GetRngStates(), ora.rng_states())
+    compare(ora, dev, sp, FP64_TOL)
+
+
+def test_full_size_batch_sites_match_single_site_oracles(gpu, oracle_mod):
+    """4096 heterogeneous sites on the device; sites are independent, so any
+    site equals a one-site oracle run with that site's channels and stream."""
+    say("full-size batch: 4096 sites")
+    sp = workloads.batched_dyads(n_sites=4096)
+    dev = DyadHIP(sp)
+    dev.InitOpeningAll()
+    ri, rb = sp.rim_arrays()
+    jj = sp.jsr_array([1000.0])
+    dev.Update(ri, rb, jj)
+    dt, n = 1e-5, 60
+    dev.Step(dt, n)
+    ca = dev.GetField(0)
+    assert np.isfinite(ca).all()
+    st = dev.GetChannelsStates()
+    for s in (0, 1234, 4095):
+        one = workloads.batched_dyads(n_sites=1)
+        one.sites = [sp.sites[s]]
+        one.rng_states = sp.rng_states[s:s + 1]
+        one.init_channel = [sp.init_channel[s]]
+        ora = oracle_mod.Oracle(one)
+        ora.init_opening()
+        r1, b1 = one.rim_arrays()
+        ora.update(r1, b1, one.jsr_array([1000.0]))
+        ora.step(dt, n, threads=1)
+        assert rel_err(ca[s], ora.field(0)[0]) < FP64_TOL
+        n_ch = len(sp.sites[s])
+        assert np.array_equal(st[s, :n_ch], ora.channel_states()[0, :n_ch])
