@@ -29,13 +29,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk):
+def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False):
+    from casim_amd import autotune
     sp = workloads.batched_dyads(n_sites=n_sites, precision=precision)
     dev = DyadHIP(sp)
     dev.InitOpeningAll()
     ri, rb = sp.rim_arrays()
     dev.Update(ri, rb, sp.jsr_array([sp.meta["jsr_uM"]]))
     dt = sp.meta["dt_ms"]
+    tuned = None if retune else autotune.apply(dev, sp)
+    if tuned is None:
+        tuned, _ = autotune.tune(dev, sp, dt, log=log)
+        # the tuning steps advanced the state: start the timed run from a reset
+        dev.Reset()
+        dev.InitOpeningAll()
+        dev.Update(ri, rb, sp.jsr_array([sp.meta["jsr_uM"]]))
+    log("tuning:", {k: tuned[k] for k in ("planes", "threads", "sweep_ms")})
     chunk = min(chunk, steps)
     rem = steps % chunk
     # warmup also captures the graphs used in the timed region
@@ -68,6 +77,7 @@ def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk):
         "sweep_gbs": n_sites * bps / (ms_sweep / steps * 1e-3) / 1e9,
         "finite": ok, "open_fraction": open_frac,
         "workload": sp.meta["workload"],
+        "tuning": {"planes": tuned["planes"], "threads": tuned["threads"]},
     }
     dev.close()
     return res
@@ -118,6 +128,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: sites are independent and this build has no multi-GPU path "
@@ -127,12 +138,12 @@ def main():
     from casim_amd.dyad import DyadHIP
     abi.load()
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
-    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk)
+    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune)
     log("fp64:", json.dumps(r64))
     r32 = None
     if not a.no_fp32:
         log("fp32 leg")
-        r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk)
+        r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune)
         log("fp32:", json.dumps(r32))
     cpu = None
     if not a.no_cpu:
@@ -155,7 +166,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": r64["workload"], "sites": a.sites, "grid": r64["grid"], "dt_ns": 10,
-                   "path": "plain batched RD, graph of %d steps" % min(a.chunk, a.steps)},
+                   "path": "plain batched RD, graph of %d steps" % min(a.chunk, a.steps),
+                   "tuning": r64["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles()},
         "cpu_baseline": cpu,

@@ -1,0 +1,80 @@
+"""Autotuner of the state sweep's launch shape.
+
+For one batch shape it times every candidate (kernel form, threads per
+workgroup, x-planes per workgroup) with the profiled step entry point, which
+brackets each sweep launch with HIP events, and keeps the fastest.  Results
+are cached in tuning.json next to this file, keyed by grid, species layout,
+precision and site count; results of the step do not depend on the choice.
+"""
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuning.json")
+
+
+def key(spec):
+    return "dim%d_%dx%dx%d_i%d_b%d_f%d_p%d_S%d" % (
+        spec.dim, spec.nx, spec.ny, spec.nz, spec.n_ions, spec.n_bound,
+        sum(int(b.faces_only) for b in spec.buffers), spec.precision, spec.n_sites)
+
+
+def candidates(spec):
+    P = spec.ny * spec.nz
+    inner = spec.nx - 2
+    out = [(-1, 256, 0)]
+    for th in (64, 128, 192, 256, 320, 384, 448, 512):
+        slots = (P + th - 1) // th
+        if slots > 4 or th > max(64, ((P + 63) // 64) * 64):
+            continue
+        if slots > 1 and th * slots - P >= th // 2:
+            continue  # more than half a block idle on the last slot
+        for chunk in sorted({0, (inner + 1) // 2, (inner + 3) // 4}):
+            for pf in ((1, 2, 3) if slots <= 2 else (1,)):
+                out.append((chunk, th, pf))
+    return out
+
+
+def load_cache():
+    if os.path.exists(CACHE):
+        try:
+            with open(CACHE) as f:
+                return json.load(f)
+        except ValueError:
+            return {}
+    return {}
+
+
+def lookup(spec):
+    return load_cache().get(key(spec))
+
+
+def apply(dev, spec):
+    """Apply the cached tuning of this shape to a DyadHIP handle, if any."""
+    t = lookup(spec)
+    if t:
+        dev.SetTuning(t["planes"], t["threads"], t.get("prefetch", 1))
+    return t
+
+
+def tune(dev, spec, dt, steps=20, warmup=4, save=True, log=None):
+    """Time every candidate on `dev` (a DyadHIP after its first Update)."""
+    res = []
+    for planes, th, pf in candidates(spec):
+        try:
+            dev.SetTuning(planes, th, pf)
+        except RuntimeError:
+            continue
+        dev.StepProfiled(dt, warmup)
+        ms, _ = dev.StepProfiled(dt, steps)
+        res.append({"planes": planes, "threads": th, "prefetch": pf, "sweep_ms": ms / steps})
+        if log:
+            log("tune planes=%d threads=%d prefetch=%d: %.4f ms" % (planes, th, pf, ms / steps))
+    best = min(res, key=lambda r: r["sweep_ms"])
+    dev.SetTuning(best["planes"], best["threads"], best["prefetch"])
+    if save:
+        c = load_cache()
+        c[key(spec)] = dict(best, all=res)
+        with open(CACHE, "w") as f:
+            json.dump(c, f, indent=1, sort_keys=True)
+    return best, res

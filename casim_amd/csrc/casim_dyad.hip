@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/casim_dyad.h"
+#include "dyad_internal.hpp"
 #include "dyad_kernels.hpp"
 #include "mrg32k3a.hpp"
 
@@ -25,29 +26,13 @@ namespace {
 
 thread_local std::string g_err;
 
-struct Err : std::runtime_error {
-    int code;
-    Err(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCK(x)                                                                                  \
-    do {                                                                                          \
-        hipError_t e_ = (x);                                                                      \
-        if (e_ != hipSuccess) throw Err(CASIM_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define REQUIRE(cond, code, msg)        \
-    do {                                \
-        if (!(cond)) throw Err(code, msg); \
-    } while (0)
-
 template <class F>
 int guard(F &&f) {
     try {
         f();
         g_err.clear();
         return CASIM_OK;
-    } catch (Err &e) {
+    } catch (casim::Err &e) {
         g_err = e.what();
         return e.code;
     } catch (std::exception &e) {
@@ -104,43 +89,6 @@ T *dalloc(size_t n) {
 
 }  // namespace
 
-struct CasimDyad {
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    Params P{};
-    Channels Ch{};
-    int precision = CASIM_FP64;
-    size_t tsize = 8;
-    void *g[2] = {nullptr, nullptr};
-    void *f[2] = {nullptr, nullptr};
-    int cur = 0;
-    int nch = 0, M = 0;
-    double zext = 0, dx = 0, dy = 0, dz = 0, radius = 0, voltage = 0;
-    double elem_volume = 0, grad_mult = 0, sxm = 0, sym = 0;
-    std::vector<int> h_off, h_node, h_perm, h_snode;
-    std::vector<uint32_t> h_rng0;
-    int *d_off = nullptr, *d_node = nullptr, *d_type = nullptr, *d_perm = nullptr, *d_snode = nullptr,
-        *d_state = nullptr, *d_chans = nullptr;
-    uint32_t *d_rng = nullptr, *d_rng0 = nullptr, *d_jump = nullptr;
-    double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
-           *d_rim_bound = nullptr, *d_eff = nullptr;
-    bool have_update = false;
-    int threads = 256;
-    std::map<std::tuple<int, double, int>, hipGraphExec_t> graphs;
-
-    size_t grid_elems() const { return (size_t)P.n_grid * P.S * P.NS; }
-    size_t face_elems() const { return (size_t)P.n_face * P.S * 2 * P.nFP; }
-
-    void use() { HIPCK(hipSetDevice(dev)); }
-
-    void launch_step(double dt);
-    void launch_channels(int mode, double dt);
-    void launch_efflux();
-    void clear_graphs() {
-        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
-    }
-};
 
 namespace {
 
@@ -159,6 +107,10 @@ void step_launch(CasimDyad *d, double dt) {
 template <typename T>
 void step_dispatch(CasimDyad *d, double dt) {
     const Params &p = d->P;
+    if (d->use_sweep && d->sweep_threads > 0) {
+        if (sizeof(T) == 8) return p.dim == 3 ? sweep_launch_f64_3d(d, dt) : sweep_launch_f64_2d(d, dt);
+        return p.dim == 3 ? sweep_launch_f32_3d(d, dt) : sweep_launch_f32_2d(d, dt);
+    }
     if (p.dim == 3) {
         if (p.ni == 1 && p.nb == 3) return step_launch<T, 3, 1, 3>(d, dt);
         if (p.ni == 2 && p.nb == 4) return step_launch<T, 3, 2, 4>(d, dt);
@@ -317,6 +269,9 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         P.n_grid = P.ni;
         P.n_face = 0;
         for (int b = 0; b < P.nb; ++b) P.bslot[b] = P.bface[b] ? P.n_face++ : P.n_grid++;
+        for (int j = 0; j < P.ni; ++j) P.gD[j] = P.Dion[j];
+        for (int b = 0; b < P.nb; ++b)
+            if (!P.bface[b]) P.gD[P.bslot[b]] = P.Db[b];
         // equilibrium (DyadRDCPU.cpp:53-81)
         for (int b = 0; b < P.nb; ++b) P.binit[b] = 0.0;
         if (c->start_equilibrium) {
@@ -350,6 +305,16 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         d->sym = c->dim == 3 ? c->dx * c->dz / c->dy : c->dx * c->z / c->dy;
         d->precision = c->precision;
         d->tsize = c->precision == CASIM_FP64 ? 8 : 4;
+        {
+            // plane-marching sweep: one plane of every grid field in LDS (x2)
+            const int Pl = P.ny * P.nz;
+            const size_t lds = 2 * (size_t)P.n_grid * Pl * d->tsize;
+            const int th = Pl > 256 ? 256 : (Pl + 63) / 64 * 64;
+            if (Pl <= 4 * 512 && lds <= 65536 && (Pl + th - 1) / th <= 4) {
+                d->sweep_threads = th;
+                d->sweep_chunk = P.nx - 2;
+            }
+        }
 
         // channels
         Channels &Ch = d->Ch;
@@ -893,13 +858,29 @@ double casim_dyad_bytes_per_site_step(const CasimDyad *d) {
     return (double)d->tsize * (P.n_grid * (all + inner) + P.n_face * (fall + finner));
 }
 
-int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block) {
+int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block, int32_t prefetch) {
     return guard([&] {
         REQUIRE(d, CASIM_E_INVALID, "null handle");
-        (void)planes_per_block;
-        int th = threads_per_block ? threads_per_block : 256;
-        REQUIRE(th % 64 == 0 && th >= 64 && th <= 256, CASIM_E_INVALID, "threads must be 64..256, multiple of 64");
-        d->threads = th;
+        const Params &P = d->P;
+        const int Pl = P.ny * P.nz;
+        if (planes_per_block < 0) {  // thread-per-node kernel
+            int th = threads_per_block ? threads_per_block : 256;
+            REQUIRE(th % 64 == 0 && th >= 64 && th <= 256, CASIM_E_INVALID,
+                    "threads must be 64..256, multiple of 64");
+            d->threads = th;
+            d->use_sweep = false;
+        } else {
+            REQUIRE(d->sweep_threads > 0, CASIM_E_INVALID, "the plane-marching sweep does not fit this grid");
+            int th = threads_per_block ? threads_per_block : d->sweep_threads;
+            REQUIRE(th % 64 == 0 && th >= 64 && th <= 512, CASIM_E_INVALID,
+                    "threads must be 64..512, multiple of 64");
+            REQUIRE((Pl + th - 1) / th <= 4, CASIM_E_INVALID, "too few threads for one plane (at most 4 slots each)");
+            d->sweep_threads = th;
+            d->sweep_chunk = planes_per_block ? std::min(planes_per_block, P.nx - 2) : P.nx - 2;
+            REQUIRE(prefetch >= 0 && prefetch <= 3, CASIM_E_INVALID, "prefetch depth must be 0..3");
+            d->sweep_prefetch = prefetch ? prefetch : 1;
+            d->use_sweep = true;
+        }
         d->clear_graphs();
     });
 }

@@ -1,0 +1,83 @@
+// dyad_internal.hpp -- the handle behind the C ABI and the error helpers,
+// shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/casim_dyad.h"
+#include "dyad_kernels.hpp"
+
+namespace casim {
+
+struct Err : std::runtime_error {
+    int code;
+    Err(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+}  // namespace casim
+
+#define HIPCK(x)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) throw casim::Err(CASIM_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define REQUIRE(cond, code, msg)               \
+    do {                                       \
+        if (!(cond)) throw casim::Err(code, msg); \
+    } while (0)
+
+struct CasimDyad {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    casim::Params P{};
+    casim::Channels Ch{};
+    int precision = CASIM_FP64;
+    size_t tsize = 8;
+    void *g[2] = {nullptr, nullptr};
+    void *f[2] = {nullptr, nullptr};
+    int cur = 0;
+    int nch = 0, M = 0;
+    double zext = 0, dx = 0, dy = 0, dz = 0, radius = 0, voltage = 0;
+    double elem_volume = 0, grad_mult = 0, sxm = 0, sym = 0;
+    std::vector<int> h_off, h_node, h_perm, h_snode;
+    std::vector<uint32_t> h_rng0;
+    int *d_off = nullptr, *d_node = nullptr, *d_type = nullptr, *d_perm = nullptr, *d_snode = nullptr,
+        *d_state = nullptr, *d_chans = nullptr;
+    uint32_t *d_rng = nullptr, *d_rng0 = nullptr, *d_jump = nullptr;
+    double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
+           *d_rim_bound = nullptr, *d_eff = nullptr;
+    bool have_update = false;
+    int threads = 256;        // k_step block size
+    int sweep_threads = 0;    // k_sweep block size (0: not usable)
+    int sweep_chunk = 0;      // interior x-planes per k_sweep workgroup
+    int sweep_prefetch = 1;   // planes prefetched beyond x+1
+    bool use_sweep = true;
+    std::map<std::tuple<int, double, int>, hipGraphExec_t> graphs;
+
+    size_t grid_elems() const { return (size_t)P.n_grid * P.S * P.NS; }
+    size_t face_elems() const { return (size_t)P.n_face * P.S * 2 * P.nFP; }
+
+    void use() { HIPCK(hipSetDevice(dev)); }
+
+    void launch_step(double dt);
+    void launch_channels(int mode, double dt);
+    void launch_efflux();
+    void clear_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+};
+
+namespace casim {
+// State-sweep launchers, one translation unit each (sweep_inst.hip).
+void sweep_launch_f64_3d(CasimDyad *d, double dt);
+void sweep_launch_f64_2d(CasimDyad *d, double dt);
+void sweep_launch_f32_3d(CasimDyad *d, double dt);
+void sweep_launch_f32_2d(CasimDyad *d, double dt);
+}  // namespace casim

@@ -38,6 +38,7 @@ struct Params {
     int sl[MAXI];
     double N1[MAXI], K1[MAXI], N2[MAXI], K2[MAXI];
     double c0[MAXI];
+    double gD[MAXI + MAXB];  // diffusion coefficient of grid field g
     double ihx2, ihy2, ihz2;
 };
 
@@ -161,17 +162,18 @@ __global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__re
             }
         }
     }
-    // free buffers from the bound fields, subtracted in bound order
-    T fr[MAXU];
-#pragma unroll
-    for (int u = 0; u < MAXU; ++u) {
-        if (u >= p.nu) break;
-        fr[u] = (T)total_at(p, u, z);
-    }
+    // free buffer seen by each bound field, subtracted in bound order
+    T frb[LB];
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
         if (!NB && b >= p.nb) break;
-        fr[p.bbuf[b]] -= B[b];
+        T v = (T)total_at(p, p.bbuf[b], z);
+#pragma unroll
+        for (int b2 = 0; b2 < LB; ++b2) {
+            if (!NB && b2 >= p.nb) break;
+            if (p.bbuf[b2] == p.bbuf[b]) v -= B[b2];
+        }
+        frb[b] = v;
     }
     // reactions R1 = koff B - kon C Bfree
 #pragma unroll
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__re
             if (!NB && b >= p.nb) break;
             if (p.bion[b] != j) continue;
             if (p.bface[b] && !face) continue;
-            T R1 = (T)p.koff[b] * B[b] - (T)p.kon[b] * C[j] * fr[p.bbuf[b]];
+            T R1 = (T)p.koff[b] * B[b] - (T)p.kon[b] * C[j] * frb[b];
             evoB[b] += -R1;
             R += R1;
         }
@@ -221,6 +223,255 @@ __global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__re
         } else {
             gout[p.bslot[b] * fs + c] = B[b] + evoB[b] * dt;
         }
+    }
+}
+
+
+// Workgroup barrier that orders LDS traffic only: global loads issued before
+// it stay in flight (the plane prefetch below relies on that).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// ---------------------------------------------------------------------------
+// The state sweep, plane-marching form.  One workgroup owns one x-chunk of one
+// site and walks its planes x0..x1-1 in order.  A plane (fixed x) is the
+// contiguous [y][z] block of P = ny*nz nodes of every field; thread t owns the
+// plane slots q = t + k*blockDim (k < SL) in every plane, so the x-neighbours
+// of its nodes are its own registers (planes x-1, x, x+1 held in A, B, C; plane
+// x+2 prefetched into N while x is computed).  Only the current plane goes
+// through LDS (double-buffered, one LDS-only barrier per plane) for the y and
+// z neighbours.  Every state value is read from HBM once per chunk; chunks add
+// two halo planes.  Units are laid out so that consecutive chunks of a site
+// land on the same XCD (blocks are dealt to XCDs round-robin).
+// Arithmetic and operation order as k_step (DyadRDCPU.cpp:404-506).
+template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int SL, int PD>
+__global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__restrict__ gout,
+                                               const T *__restrict__ fin, T *__restrict__ fout,
+                                               const int *__restrict__ ch_off, const int *__restrict__ snode,
+                                               const double *__restrict__ sJ, int nch, Params p, double dt_,
+                                               int chunk, int nchunks, int units) {
+    constexpr int LI = NI ? NI : MAXI;
+    constexpr int LB = NB ? NB : MAXB;
+    constexpr int NG = NGT ? NGT : LI + LB;         // grid fields (upper bound when NGT = 0)
+    constexpr int NF = NGT ? (NFT ? NFT : 1) : LB;  // face fields (upper bound when NGT = 0)
+    constexpr int RS = 3 + PD;                      // register ring: planes x-1 .. x+1+PD
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int per = gridDim.x >> 3;
+    const int unit = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (unit >= units) return;
+    const int s = unit / nchunks;
+    const int cidx = unit - s * nchunks;
+    const int x0 = 1 + cidx * chunk;
+    const int x1 = min(p.nx - 1, x0 + chunk);
+    const int P = p.ny * p.nz;
+    const int ng = p.n_grid, nfc = p.n_face;
+    const size_t fs = (size_t)p.S * p.NS;
+    const T *gs = gin + (size_t)s * p.NS;
+    T *os = gout + (size_t)s * p.NS;
+    const T dt = (T)dt_;
+    const T ihx2 = (T)p.ihx2, ihy2 = (T)p.ihy2, ihz2 = (T)p.ihz2;
+
+    int q[SL], zl[SL], zr[SL], fz[SL], yy[SL];
+    bool act[SL], inner[SL];
+#pragma unroll
+    for (int k = 0; k < SL; ++k) {
+        q[k] = threadIdx.x + k * blockDim.x;
+        act[k] = q[k] < P;
+        const int z = DIM == 3 ? q[k] % p.nz : 0;
+        yy[k] = DIM == 3 ? q[k] / p.nz : q[k];
+        inner[k] = act[k] && yy[k] >= 1 && yy[k] <= p.ny - 2;
+        zl[k] = DIM == 3 ? (z == 0 ? 1 : -1) : 0;
+        zr[k] = DIM == 3 ? (z == p.nz - 1 ? -1 : 1) : 0;
+        fz[k] = (DIM == 3 && act[k]) ? (z == 0 ? 0 : (z == p.nz - 1 ? 1 : -1)) : -1;
+    }
+
+    // R[r] holds plane x-1+r of every grid field; F[r] the face values of plane x+r
+    T R[RS][SL][NG];
+    T F[PD + 1][SL][NF];
+    auto load_plane = [&](T (&D)[SL][NG], int x) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng && act[k]) D[k][g] = gs[g * fs + (size_t)x * P + q[k]];
+    };
+    auto load_face = [&](T (&D)[SL][NF], int x) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f < nfc && fz[k] >= 0)
+                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)fz[k] * p.nFP + (size_t)x * p.ny + yy[k]];
+    };
+#pragma unroll
+    for (int r = 0; r < RS - 1; ++r)
+        if (x0 - 1 + r <= x1) load_plane(R[r], x0 - 1 + r);
+#pragma unroll
+    for (int r = 0; r < PD; ++r)
+        if (x0 + r < x1) load_face(F[r], x0 + r);
+
+    // first mouth of this chunk in the node-sorted list
+    int cur = ch_off[s];
+    const int chi = ch_off[s + 1];
+    {
+        int e = chi;
+        const int key = x0 * P;
+        while (cur < e) {
+            const int m = (cur + e) >> 1;
+            if (snode[m] < key) cur = m + 1; else e = m;
+        }
+    }
+
+    for (int x = x0; x < x1; ++x) {
+        if (x + 1 + PD <= x1) load_plane(R[RS - 1], x + 1 + PD);
+        if (x + PD < x1) load_face(F[PD], x + PD);
+        T *L = lds + (size_t)((x - x0) & 1) * ng * P;
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng && act[k]) L[g * P + q[k]] = R[1][k][g];
+        lds_barrier();
+
+        // channel currents on this plane (the last channel on a node wins)
+        T J[SL][LI];
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int j = 0; j < LI; ++j) J[k][j] = 0;
+        const int pend = (x + 1) * P;
+        while (cur < chi) {
+            const int nd = snode[cur];
+            if (nd >= pend) break;
+            const int qq = nd - x * P;
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+                if (q[k] == qq)
+#pragma unroll
+                    for (int j = 0; j < LI; ++j)
+                        if (NI || j < p.ni) J[k][j] = (T)sJ[(size_t)j * nch + cur];
+            ++cur;
+        }
+
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+            if (!inner[k]) continue;
+            const int c = q[k];
+            const bool face = fz[k] >= 0;
+            const int z = DIM == 3 ? c % p.nz : 0;
+            T Cv[LI], eC[LI], Bv[LB], eB[LB];
+#pragma unroll
+            for (int j = 0; j < LI; ++j) {
+                if (!NI && j >= p.ni) break;
+                const T vc = R[1][k][j];
+                Cv[j] = vc;
+                const T *u = L + j * P + c;
+                T lap = (R[2][k][j] - 2 * vc + R[0][k][j]) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
+                if (DIM == 3) lap = lap + (u[zr[k]] - 2 * vc + u[zl[k]]) * ihz2;
+                eC[j] = (T)p.Dion[j] * lap;
+            }
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                if (!NB && b >= p.nb) break;
+                const int g = p.bslot[b];
+                if (p.bface[b]) {
+                    T fv = (T)0;
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        if (f == g) fv = F[0][k][f];
+                    Bv[b] = face ? fv : (T)0;
+                    eB[b] = 0;
+                } else {
+                    T vc = (T)0, cxr = (T)0, cxl = (T)0;
+#pragma unroll
+                    for (int gg = 0; gg < NG; ++gg)
+                        if (gg == g) {
+                            vc = R[1][k][gg];
+                            cxr = R[2][k][gg];
+                            cxl = R[0][k][gg];
+                        }
+                    Bv[b] = vc;
+                    if (p.Db[b] != 0.0) {
+                        const T *u = L + g * P + c;
+                        T lap = (cxr - 2 * vc + cxl) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
+                        if (DIM == 3) lap = lap + (u[zr[k]] - 2 * vc + u[zl[k]]) * ihz2;
+                        eB[b] = (T)p.Db[b] * lap;
+                    } else {
+                        eB[b] = 0;
+                    }
+                }
+            }
+            // free buffer seen by each bound field: total minus its buffer's
+            // bound fields, subtracted in bound order
+            T frb[LB];
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                if (!NB && b >= p.nb) break;
+                T v = (T)total_at(p, p.bbuf[b], z);
+#pragma unroll
+                for (int b2 = 0; b2 < LB; ++b2) {
+                    if (!NB && b2 >= p.nb) break;
+                    if (p.bbuf[b2] == p.bbuf[b]) v -= Bv[b2];
+                }
+                frb[b] = v;
+            }
+#pragma unroll
+            for (int j = 0; j < LI; ++j) {
+                if (!NI && j >= p.ni) break;
+                T Rr = 0;
+#pragma unroll
+                for (int b = 0; b < LB; ++b) {
+                    if (!NB && b >= p.nb) break;
+                    if (p.bion[b] != j) continue;
+                    if (p.bface[b] && !face) continue;
+                    T R1 = (T)p.koff[b] * Bv[b] - (T)p.kon[b] * Cv[j] * frb[b];
+                    eB[b] += -R1;
+                    Rr += R1;
+                }
+                eC[j] += Rr;
+            }
+            const size_t o = (size_t)x * P + c;
+#pragma unroll
+            for (int j = 0; j < LI; ++j) {
+                if (!NI && j >= p.ni) break;
+                T e = eC[j] + J[k][j];
+                if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
+                    T s1 = (T)p.K1[j] + Cv[j];
+                    T s2 = (T)p.K2[j] + Cv[j];
+                    e *= (T)1 / ((T)1 + (T)p.N1[j] * (T)p.K1[j] / (s1 * s1) + (T)p.N2[j] * (T)p.K2[j] / (s2 * s2));
+                }
+                os[j * fs + o] = Cv[j] + e * dt;
+            }
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                if (!NB && b >= p.nb) break;
+                if (p.bface[b]) {
+                    if (face)
+                        fout[((size_t)p.bslot[b] * p.S + s) * 2 * p.nFP + (size_t)fz[k] * p.nFP + (size_t)x * p.ny +
+                             yy[k]] = Bv[b] + eB[b] * dt;
+                } else {
+                    os[p.bslot[b] * fs + o] = Bv[b] + eB[b] * dt;
+                }
+            }
+        }
+        // advance the rings by one plane
+#pragma unroll
+        for (int r = 0; r < RS - 1; ++r)
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+#pragma unroll
+                for (int g = 0; g < NG; ++g) R[r][k][g] = R[r + 1][k][g];
+#pragma unroll
+        for (int r = 0; r < PD; ++r)
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) F[r][k][f] = F[r + 1][k][f];
     }
 }
 
@@ -416,7 +667,7 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
 
 // DyadChannels::InitOpening (DyadChannels.cpp:83-95): n+1 draws, all closed,
 // then `channel` open.  One thread per site.
-__global__ void k_init_opening(Channels a, int S, int site, int channel, const int *__restrict__ chans) {
+static __global__ void k_init_opening(Channels a, int S, int site, int channel, const int *__restrict__ chans) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || (site >= 0 && s != site)) return;
     const int off = a.off[s], n = a.off[s + 1] - off;

@@ -142,8 +142,8 @@ class DyadHIP:
     def BytesPerSiteStep(self):
         return self.lib.casim_dyad_bytes_per_site_step(self.h)
 
-    def SetTuning(self, planes=0, threads=0):
-        abi.check(self.lib.casim_dyad_set_tuning(self.h, planes, threads))
+    def SetTuning(self, planes=0, threads=0, prefetch=0):
+        abi.check(self.lib.casim_dyad_set_tuning(self.h, planes, threads, prefetch))
 
     def stream(self):
         return self.lib.casim_dyad_stream(self.h)

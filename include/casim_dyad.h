@@ -235,9 +235,13 @@ int32_t casim_dyad_channel_node(const CasimDyad *d, int32_t site, int32_t ch);
  * stored node once, write every updated node once). */
 double casim_dyad_bytes_per_site_step(const CasimDyad *d);
 
-/* Tuning of the state-sweep kernel: planes per workgroup along x (0 = auto)
- * and threads per workgroup (multiple of 64; 0 = auto). */
-int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block);
+/* Tuning of the state sweep.  planes_per_block >= 0 selects the
+ * plane-marching kernel with that many interior x-planes per workgroup
+ * (0 = a whole site per workgroup); -1 selects the thread-per-node kernel.
+ * threads_per_block: multiple of 64 (0 = default).  prefetch: planes loaded
+ * ahead of the one being updated, 1..3 (0 = 1).  Results do not depend on
+ * the tuning. */
+int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block, int32_t prefetch);
 
 /* HIP stream of the handle (hipStream_t), for callers that time with events. */
 void *casim_dyad_stream(CasimDyad *d);

@@ -220,3 +220,28 @@ def test_full_size_batch_sites_match_single_site_oracles(gpu, oracle_mod):
         assert rel_err(ca[s], ora.field(0)[0]) < FP64_TOL
         n_ch = len(sp.sites[s])
         assert np.array_equal(st[s, :n_ch], ora.channel_states()[0, :n_ch])
+
+
+@pytest.mark.parametrize("planes,threads,prefetch", [(-1, 256, 0), (0, 128, 1), (0, 448, 1), (10, 256, 2),
+                                                     (20, 192, 1), (0, 448, 3), (20, 256, 3)])
+def test_sweep_tunings_match_oracle(gpu, oracle_mod, planes, threads, prefetch):
+    """Every launch shape of the state sweep gives the oracle's results."""
+    sp = workloads.batched_dyads(n_sites=3, geometry_seed=11)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dev.SetTuning(planes, threads, prefetch)
+    dt = 1e-5
+    dev.Step(dt, 150)
+    ora.step(dt, 150)
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+    compare(ora, dev, sp, FP64_TOL)
+
+
+def test_sweep_2d_slots(gpu, oracle_mod):
+    """2D rows longer than one block: several plane slots per thread."""
+    from tests.helpers import spec
+    sp = spec(dim=2, nx=40, ny=150, h=2.0, clamped=False, channels=[(40.0, 100.0), (46.0, 100.0)], n_sites=2)
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dev.SetTuning(0, 64)  # 150 nodes per row over 64 threads: 3 slots
+    dev.Step(2e-6, 200)  # D dt (2/h^2 + 2/h^2) = 0.44: inside the Euler limit
+    ora.step(2e-6, 200)
+    compare(ora, dev, sp, FP64_TOL)
