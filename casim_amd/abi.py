@@ -23,6 +23,18 @@ _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint64)
 
 
+class CasimPoolConfig(C.Structure):
+    _fields_ = [
+        ("jsr_constant", C.c_int32), ("jsr_cq_total", C.c_double), ("jsr_volume", C.c_double),
+        ("jsr_c0", _dp), ("jsr_cq_sites", _dp), ("jsr_k_cq", _dp), ("jsr_tau_refill", _dp),
+        ("nsr_c", _dp),
+        ("cytosol_pool", C.c_int32), ("cyto_leak_mult", C.c_double), ("n_cyto_buffers", C.c_int32),
+        ("cyto_buffer_total", _dp), ("cyto_buffer_D", _dp), ("cyto_buffer_kon", _dp), ("cyto_buffer_koff", _dp),
+        ("cyto_buffer_binds", _ip),
+        ("serca_binds", _ip), ("serca_jmax", _dp), ("serca_kup", _dp),
+    ]
+
+
 class CasimDyadConfig(C.Structure):
     _fields_ = [
         ("dim", C.c_int32),
@@ -85,6 +97,7 @@ def _declare(lib):
         "casim_dyad_get_channel_states": (C.c_int, [P, _ip]),
         "casim_dyad_set_channel_states": (C.c_int, [P, _ip]),
         "casim_dyad_get_mouths": (C.c_int, [P, _dp]),
+        "casim_dyad_get_release_stats": (C.c_int, [P, _dp]),
         "casim_dyad_get_rng_states": (C.c_int, [P, _up]),
         "casim_dyad_set_rng_states": (C.c_int, [P, _up]),
         "casim_dyad_element_volume": (C.c_double, [P]),

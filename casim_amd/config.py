@@ -279,3 +279,89 @@ def load_json(path, n_sites=1, **kw):
     with open(path) as f:
         j = json.load(f)
     return from_json(j["Dyad"], n_sites=n_sites, base_dir=os.path.dirname(os.path.abspath(path)), **kw), j
+
+
+@dataclass
+class PoolSpec:
+    """jSR pool, network SR and cytosol around the dyad (SRPool.cpp:4-53,
+    NSRConstPool.cpp:3-14, CytosolPool.cpp:3-157), over the dyad's ions."""
+    jsr_constant: bool
+    jsr_cq_total: float
+    jsr_volume: float
+    jsr_c0: List[float]
+    jsr_cq_sites: List[float]
+    jsr_k_cq: List[float]
+    jsr_tau_refill: List[float]
+    nsr_c: List[float]
+    cytosol_pool: bool = False
+    cyto_leak_mult: float = 0.0
+    cyto_buffers: List[Buffer] = field(default_factory=list)
+    serca: Dict[str, Tuple[float, float]] = field(default_factory=dict)  # ion -> (Jmax, Kup)
+
+    def to_struct(self, ions):
+        ni = len(ions)
+        names = [i.name for i in ions]
+        keep = {}
+
+        def arr(name, values, dtype):
+            a = np.ascontiguousarray(np.array(values, dtype=dtype).reshape(-1))
+            if a.size == 0:
+                a = np.zeros(1, dtype=dtype)
+            keep[name] = a
+            return a
+
+        c = abi.CasimPoolConfig()
+        c.jsr_constant = int(self.jsr_constant)
+        c.jsr_cq_total = self.jsr_cq_total
+        c.jsr_volume = self.jsr_volume
+        for nm in ("jsr_c0", "jsr_cq_sites", "jsr_k_cq", "jsr_tau_refill", "nsr_c"):
+            setattr(c, nm, abi.dptr(arr(nm, getattr(self, nm), np.float64)))
+        c.cytosol_pool = int(self.cytosol_pool)
+        c.cyto_leak_mult = self.cyto_leak_mult
+        nb = len(self.cyto_buffers)
+        c.n_cyto_buffers = nb
+        c.cyto_buffer_total = abi.dptr(arr("cbt", [b.total for b in self.cyto_buffers], np.float64))
+        D = np.zeros((nb, ni))
+        kon = np.zeros((nb, ni))
+        koff = np.zeros((nb, ni))
+        binds = np.zeros((nb, ni), dtype=np.int32)
+        for u, b in enumerate(self.cyto_buffers):
+            for ion, (d_, kn, kf) in b.kinetics.items():
+                if ion in names:
+                    k = names.index(ion)
+                    D[u, k], kon[u, k], koff[u, k], binds[u, k] = d_, kn, kf, 1
+        c.cyto_buffer_D = abi.dptr(arr("cbD", D, np.float64))
+        c.cyto_buffer_kon = abi.dptr(arr("cbkon", kon, np.float64))
+        c.cyto_buffer_koff = abi.dptr(arr("cbkoff", koff, np.float64))
+        c.cyto_buffer_binds = abi.iptr(arr("cbb", binds, np.int32))
+        c.serca_binds = abi.iptr(arr("sb", [int(n in self.serca) for n in names], np.int32))
+        c.serca_jmax = abi.dptr(arr("sj", [self.serca.get(n, (0.0, 0.0))[0] for n in names], np.float64))
+        c.serca_kup = abi.dptr(arr("sk", [self.serca.get(n, (0.0, 0.0))[1] for n in names], np.float64))
+        return c, keep
+
+
+def pools_from_json(j, spec):
+    """PoolSpec from the reference's "jSR", "nSR" and "Cytosol" objects."""
+    names = [i.name for i in spec.ions]
+    jj = j["jSR"]
+    ji = jj.get("Ions", {})
+    get = lambda ion, key, dflt: float(ji[ion][key]) if ion in ji else dflt
+    nsr = j.get("nSR", {}).get("Ions", {})
+    cy = j.get("Cytosol", {})
+    if cy.get("Ions"):
+        raise ValueError("cytosol-only ions are not supported (CytosolPool indexes them as dyad effluxes)")
+    bufs = []
+    for k, v in _sorted_items(cy.get("Buffers", {})):
+        kin = {ik: (float(iv["D"]), float(iv["kon"]), float(iv["koff"])) for ik, iv in _sorted_items(v["Ions"])}
+        bufs.append(Buffer(k, float(v["Total Concentration"]), kin))
+    serca = {k: (float(v["Jmax"]), float(v["Kup"])) for k, v in _sorted_items(cy.get("SERCA", {}))}
+    return PoolSpec(
+        jsr_constant=bool(jj.get("Constant", False)), jsr_cq_total=float(jj.get("Total CQ (uM)", 0.0)),
+        jsr_volume=float(jj.get("V jSR (nm3)", 1.0)),
+        jsr_c0=[get(n, "Concentration", 0.0) for n in names],
+        jsr_cq_sites=[get(n, "Binding sites of CQ", 0.0) for n in names],
+        jsr_k_cq=[get(n, "K CQ (uM)", 0.0) for n in names],
+        jsr_tau_refill=[get(n, "T refill (ms)", 1.0) for n in names],
+        nsr_c=[float(nsr[n]["Concentration (uM)"]) if n in nsr else 0.0 for n in names],
+        cytosol_pool=cy.get("Type") == "Pool", cyto_leak_mult=float(cy.get("Leak mult", 0.0)),
+        cyto_buffers=bufs, serca=serca)

@@ -414,6 +414,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         d->d_rim_ions = dalloc<double>((size_t)P.ni * P.S);
         d->d_rim_bound = dalloc<double>((size_t)std::max(1, P.nb) * P.S);
         d->d_eff = dalloc<double>((size_t)(P.ni + P.nb) * P.S);
+        d->d_stats = dalloc<double>((size_t)NSTAT * P.S);
         auto up = [&](void *dst, const void *src, size_t bytes) {
             if (bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, d->stream));
         };
@@ -440,6 +441,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         Ch.jsr = d->d_jsr;
         Ch.sJ = d->d_sJ;
         Ch.totals = d->d_totals;
+        Ch.stats = d->d_stats;
         HIPCK(hipStreamSynchronize(d->stream));
         if (casim_dyad_reset(d) != CASIM_OK) throw Err(CASIM_E_DEVICE, g_err);
         *out = d;
@@ -459,7 +461,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     d->clear_graphs();
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
                     d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
-                    d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff};
+                    d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -482,6 +484,7 @@ int casim_dyad_reset(CasimDyad *d) {
                                (float *)d->f[0], (float *)d->f[1], P);
         HIPCK(hipGetLastError());
         HIPCK(hipMemsetAsync(d->d_state, 0, sizeof(int) * std::max(1, d->nch), d->stream));
+        HIPCK(hipMemsetAsync(d->d_stats, 0, sizeof(double) * NSTAT * P.S, d->stream));
         HIPCK(hipMemcpyAsync(d->d_rng, d->d_rng0, sizeof(uint32_t) * P.S * 6, hipMemcpyDeviceToDevice, d->stream));
         HIPCK(hipStreamSynchronize(d->stream));
         d->cur = 0;
@@ -670,6 +673,25 @@ int casim_dyad_get_total_sr_current(CasimDyad *d, double *out) {
         REQUIRE(d && out, CASIM_E_INVALID, "null argument");
         d->use();
         copy_out(d, out, d->d_totals, (size_t)d->P.ni * d->P.S);
+    });
+}
+
+int casim_dyad_get_release_stats(CasimDyad *d, double *out) {
+    return guard([&] {
+        REQUIRE(d && out, CASIM_E_INVALID, "null argument");
+        d->use();
+        std::vector<double> h((size_t)NSTAT * d->P.S);
+        copy_out(d, h.data(), d->d_stats, h.size());
+        for (int s = 0; s < d->P.S; ++s) {
+            const double *st = &h[(size_t)s * NSTAT];
+            double *o = out + (size_t)s * 6;
+            o[0] = st[ST_AMP];
+            o[1] = st[ST_TTP];
+            o[2] = st[ST_TTE];
+            o[3] = st[ST_Q];
+            o[4] = st[ST_TTE] > 0 ? st[ST_Q] / st[ST_TTE] : 0.0;
+            o[5] = st[ST_NOPEN];
+        }
     });
 }
 

@@ -51,7 +51,7 @@ struct CasimDyad {
         *d_state = nullptr, *d_chans = nullptr;
     uint32_t *d_rng = nullptr, *d_rng0 = nullptr, *d_jump = nullptr;
     double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
-           *d_rim_bound = nullptr, *d_eff = nullptr;
+           *d_rim_bound = nullptr, *d_eff = nullptr, *d_stats = nullptr;
     bool have_update = false;
     int threads = 256;        // k_step block size
     int sweep_threads = 0;    // k_sweep block size (0: not usable)

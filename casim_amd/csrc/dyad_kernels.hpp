@@ -55,6 +55,7 @@ struct Channels {
     const double *jsr;  // [M*S*ni]
     double *sJ;         // [ni*nch] currents in node-sorted order
     double *totals;     // [ni*S]
+    double *stats;      // [S*NSTAT] release statistics, see StatSlot
     int M;              // max channels per site
     int nch;            // channels over all sites
     int ca, mg;
@@ -64,6 +65,19 @@ struct Channels {
 };
 
 enum ChannelMode { CH_SNAPSHOT = 1, CH_CURRENTS = 2, CH_GATE = 4, CH_GATE_STORED = 8, CH_ACCUMULATE = 16 };
+
+// Per-site release statistics, the columns of IndChannelCREAnalyzer's
+// scenarios table (IndChannelCREAnalyzer.cpp:89-137) accumulated step by step:
+// N_j = open channels after the gating of step j, in effect from t_j = j dt.
+enum StatSlot {
+    ST_STEPS = 0,   // steps gated since InitOpening
+    ST_NOPEN = 1,   // N of the last step
+    ST_AMP = 2,     // max N
+    ST_TTP = 3,     // first time N reached its max
+    ST_TTE = 4,     // time of the last change of N
+    ST_Q = 5,       // sum of N dt
+    NSTAT = 8
+};
 
 template <typename T>
 __device__ __forceinline__ T load_face(const T *f, const Params &p, int slot, int s, int face, int xy) {
@@ -560,6 +574,7 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
 #pragma unroll
     for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
     int pos = 0;
+    int nopen = 0;
     double acc[MAXI];
 #pragma unroll
     for (int k = 0; k < MAXI; ++k)
@@ -650,12 +665,25 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
                 first = c0 + jl + 1;
             }
             if (valid) a.state[off + i] = st;
+            nopen += __popcll(__ballot(valid && st == 1));
         }
     }
     if ((mode & CH_CURRENTS) && (mode & CH_ACCUMULATE) && lane == 0) {
         for (int k = 0; k < p.ni; ++k) a.totals[s + k * p.S] = acc[k];
     }
     if ((mode & (CH_GATE | CH_GATE_STORED)) && lane == 0) {
+        double *st = a.stats + (size_t)s * NSTAT;
+        const double steps = st[ST_STEPS] + 1.0;
+        const double t = steps * dt;
+        const double nn = (double)nopen;
+        if (nn != st[ST_NOPEN]) st[ST_TTE] = t;
+        if (nn > st[ST_AMP]) {
+            st[ST_AMP] = nn;
+            st[ST_TTP] = t;
+        }
+        st[ST_Q] += nn * dt;
+        st[ST_NOPEN] = nn;
+        st[ST_STEPS] = steps;
         const uint32_t *J = a.jump + (size_t)pos * 18;
         uint32_t out[6];
         mrg::matvec(J, S6, out, mrg::C1, mrg::M1);
@@ -673,6 +701,11 @@ static __global__ void k_init_opening(Channels a, int S, int site, int channel, 
     const int off = a.off[s], n = a.off[s + 1] - off;
     const int ch = chans ? chans[s] : channel;
     for (int k = 0; k < n; ++k) a.state[off + k] = (k == ch) ? 1 : 0;
+    double *st = a.stats + (size_t)s * NSTAT;
+    const double n0 = (ch >= 0 && ch < n) ? 1.0 : 0.0;
+    for (int i = 0; i < NSTAT; ++i) st[i] = 0.0;
+    st[ST_NOPEN] = n0;
+    st[ST_AMP] = n0;
     uint32_t S6[6], out[6];
     for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
     const uint32_t *J = a.jump + (size_t)(n + 1) * 18;

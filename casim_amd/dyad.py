@@ -127,6 +127,12 @@ class DyadHIP:
         abi.check(self.lib.casim_dyad_get_mouths(self.h, abi.dptr(out)))
         return out.reshape(k, self.S, self.M)
 
+    def GetReleaseStats(self):
+        """[site, (amp, ttp, tte, Q, Q/tte, N now)] since InitOpening."""
+        out = np.empty(self.S * 6)
+        abi.check(self.lib.casim_dyad_get_release_stats(self.h, abi.dptr(out)))
+        return out.reshape(self.S, 6)
+
     def GetRngStates(self):
         out = np.empty(self.S * 6, dtype=np.uint64)
         abi.check(self.lib.casim_dyad_get_rng_states(self.h, abi.uptr(out)))

@@ -107,6 +107,38 @@ typedef struct CasimDyadConfig {
     int32_t precision;           /* CASIM_FP64 or CASIM_FP32 state grid */
 } CasimDyadConfig;
 
+/*
+ * Compartments around the dyad, for the full step of the reference's batch
+ * loop (ParallelHybridSolver.cpp:159-175, 408-437).  Arrays over ions follow
+ * the dyad's ion order.
+ */
+typedef struct CasimPoolConfig {
+    /* jSR pool (SRPool.cpp:4-124) */
+    int32_t jsr_constant;          /* "Constant": the jSR stays at jsr_c0 */
+    double jsr_cq_total;           /* "Total CQ (uM)" */
+    double jsr_volume;             /* "V jSR (nm3)" */
+    const double *jsr_c0;          /* [n_ions] "Concentration" (0 for ions the jSR lacks) */
+    const double *jsr_cq_sites;    /* [n_ions] "Binding sites of CQ" */
+    const double *jsr_k_cq;        /* [n_ions] "K CQ (uM)" */
+    const double *jsr_tau_refill;  /* [n_ions] "T refill (ms)" (1 for ions the jSR lacks) */
+    /* network SR (NSRConstPool.cpp:3-29): constant concentrations */
+    const double *nsr_c;           /* [n_ions] "Concentration (uM)" */
+    /* cytosol.  0: the rim keeps the values of the last update; 1: the
+       well-mixed pool of CytosolPool.cpp:3-365 supplies the rim and takes the
+       dyad's effluxes */
+    int32_t cytosol_pool;
+    double cyto_leak_mult;         /* "Leak mult" */
+    int32_t n_cyto_buffers;        /* buffers of the cytosol only ("Cytosol"."Buffers", sorted) */
+    const double *cyto_buffer_total;  /* [n_cyto_buffers] */
+    const double *cyto_buffer_D;   /* [n_cyto_buffers * n_ions] D of (buffer, ion), 0 if unbound */
+    const double *cyto_buffer_kon; /* [n_cyto_buffers * n_ions], kon = 0 and koff = 0 if unbound */
+    const double *cyto_buffer_koff;
+    const int32_t *cyto_buffer_binds; /* [n_cyto_buffers * n_ions] 1 if the buffer binds the ion */
+    const int32_t *serca_binds;    /* [n_ions] "SERCA" lists the ion */
+    const double *serca_jmax;      /* [n_ions] */
+    const double *serca_kup;       /* [n_ions] */
+} CasimPoolConfig;
+
 typedef struct CasimDyad CasimDyad;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -216,6 +248,18 @@ int casim_dyad_set_channel_states(CasimDyad *d, const int32_t *in);
  * (DyadRDCPU.cpp:371-390): [ch + n_ch * (site + k * S)], k over ions, bound
  * fields, free buffers. */
 int casim_dyad_get_mouths(CasimDyad *d, double *out);
+
+/* Release statistics of each site since its InitOpening, the columns of
+ * IndChannelCREAnalyzer's scenarios table (IndChannelCREAnalyzer.cpp:89-137)
+ * accumulated per step instead of per event (N = open channels after the
+ * gating of a step, in effect from that step's end):
+ *   out[s*6 + 0] amplitude  = max N
+ *   out[s*6 + 1] TTP        = first time N reached the max (ms)
+ *   out[s*6 + 2] TTE        = time of the last change of N (ms)
+ *   out[s*6 + 3] Q          = sum of N dt (ms)
+ *   out[s*6 + 4] Q / TTE    (0 while TTE = 0)
+ *   out[s*6 + 5] N now */
+int casim_dyad_get_release_stats(CasimDyad *d, double *out);
 
 /* RNG state of each site's stream (RngStream::GetState, rngstream.cpp:415-419): [site * 6]. */
 int casim_dyad_get_rng_states(CasimDyad *d, uint64_t *out);

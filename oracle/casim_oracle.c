@@ -296,6 +296,7 @@ struct OracleDyad {
     double *mouths, *gradients, *total_sr_current;
     double *rim_ions, *rim_bound, *jsr;
     int have_update;
+    int start_eq;
     double grad_mult, sx, sy;
     double *mult_z;
 };
@@ -361,6 +362,7 @@ OracleDyad *oracle_create(const CasimDyadConfig *c) {
     }
     /* Equilibrium per buffer over the ions it binds, in ion order
      * (DyadRDCPU.cpp:53-81). */
+    o->start_eq = c->start_equilibrium;
     if (c->start_equilibrium) {
         for (int u = 0; u < nb; ++u) {
             double kon[16], koff[16], cc[16], out[16];
@@ -825,3 +827,309 @@ void oracle_get_rng_states(const OracleDyad *o, uint64_t *out) {
 void oracle_set_rng_states(OracleDyad *o, const uint64_t *in) {
     for (int i = 0; i < o->S * 6; ++i) o->rng[i] = (double)in[i];
 }
+
+/* ------------------------------------------------------------------------- */
+/* Compartments around the dyad and the batch loop of ParallelHybridSolver.  */
+
+struct OracleHybrid {
+    OracleDyad *o;
+    int S, ni, M, nB; /* nB: dyad bound fields */
+    int ref_indexing; /* 1: SRPool's reads exactly as written (index mix-ups for S > 1) */
+    /* SRPool (SRPool.cpp:4-124) */
+    int jsr_const;
+    double cq, Vjsr, Velem;
+    double *nCQ, *KCQ, *tau, *jc0;
+    double *jions, *jnew, *jnsr, *IjSR; /* [M*S*ni], [M*S*ni], [ni*S], [ni*S] */
+    double *nsr;                        /* NSRConstPool output [site + ion*S] */
+    /* CytosolPool (CytosolPool.cpp:3-365) */
+    int cyto;
+    int nbc, nBc; /* unique buffers (dyad first, then cytosol), bound fields */
+    short *cbinds; /* [ni * nbc] -> bound index or -1 */
+    int *cb_dyad;  /* [nBc] 1 if the buffer is a dyad buffer */
+    double *ctot, *ckon, *ckoff, *conet_b, *cinit, *conet_i, *cbg;
+    int *serca;
+    double *jmax, *kup;
+    double *cions, *cbuf, *cfree, *cevo_i, *cevo_b, *cbb; /* cbb: boundary buffers [site + b*S] */
+    double *rim_ions, *rim_bound;                         /* constant rim when there is no pool */
+};
+
+OracleHybrid *oracle_hybrid_create(OracleDyad *o, const CasimPoolConfig *pc, const double *rim_ions,
+                                   const double *rim_bound, int ref_indexing) {
+    OracleHybrid *h = (OracleHybrid *)xcalloc(1, sizeof *h);
+    int S = o->S, ni = o->n_ions, M = o->nch_max, nB = o->n_bound, nb = o->n_buf;
+    h->o = o;
+    h->S = S;
+    h->ni = ni;
+    h->M = M;
+    h->nB = nB;
+    h->ref_indexing = ref_indexing;
+    h->jsr_const = pc->jsr_constant;
+    h->cq = pc->jsr_cq_total;
+    h->Vjsr = pc->jsr_volume;
+    h->Velem = oracle_element_volume(o);
+    h->nCQ = (double *)xcalloc(ni, sizeof(double));
+    h->KCQ = (double *)xcalloc(ni, sizeof(double));
+    h->tau = (double *)xcalloc(ni, sizeof(double));
+    h->jc0 = (double *)xcalloc(ni, sizeof(double));
+    for (int i = 0; i < ni; ++i) {
+        h->nCQ[i] = pc->jsr_cq_sites[i];
+        h->KCQ[i] = pc->jsr_k_cq[i];
+        h->tau[i] = pc->jsr_tau_refill[i];
+        h->jc0[i] = pc->jsr_c0[i];
+    }
+    h->jions = (double *)xcalloc((size_t)M * S * ni, sizeof(double));
+    h->jnew = (double *)xcalloc((size_t)M * S * ni, sizeof(double));
+    h->jnsr = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    h->IjSR = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    h->nsr = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    for (int i = 0; i < ni; ++i)
+        for (int s = 0; s < S; ++s) {
+            h->nsr[s + i * S] = pc->nsr_c[i];
+            h->jnsr[s + i * S] = h->jc0[i]; /* SRPool's ctor starts ions_NSR at the jSR value (SRPool.cpp:67-72) */
+        }
+    h->cyto = pc->cytosol_pool;
+    h->rim_ions = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    h->rim_bound = (double *)xcalloc((size_t)(nB ? nB : 1) * S, sizeof(double));
+    if (rim_ions) memcpy(h->rim_ions, rim_ions, sizeof(double) * ni * S);
+    if (rim_bound && nB) memcpy(h->rim_bound, rim_bound, sizeof(double) * nB * S);
+    /* cytosol species: dyad buffers then cytosol buffers, bound ion-major */
+    int nc = pc->cytosol_pool ? pc->n_cyto_buffers : 0;
+    h->nbc = nb + nc;
+    h->cbinds = (short *)xcalloc((size_t)ni * (h->nbc ? h->nbc : 1), sizeof(short));
+    int cap = nB + nc * ni + 1;
+    h->cb_dyad = (int *)xcalloc(cap, sizeof(int));
+    h->ckon = (double *)xcalloc(cap, sizeof(double));
+    h->ckoff = (double *)xcalloc(cap, sizeof(double));
+    h->conet_b = (double *)xcalloc(cap, sizeof(double));
+    h->cinit = (double *)xcalloc(cap, sizeof(double));
+    h->ctot = (double *)xcalloc(h->nbc + 1, sizeof(double));
+    h->conet_i = (double *)xcalloc(ni, sizeof(double));
+    h->cbg = (double *)xcalloc(ni, sizeof(double));
+    h->serca = (int *)xcalloc(ni, sizeof(int));
+    h->jmax = (double *)xcalloc(ni, sizeof(double));
+    h->kup = (double *)xcalloc(ni, sizeof(double));
+    for (int u = 0; u < nb; ++u) h->ctot[u] = o->buf_total[u];
+    for (int u = 0; u < nc; ++u) h->ctot[nb + u] = pc->cyto_buffer_total[u];
+    int k = 0;
+    double D[256];
+    for (int j = 0; j < ni; ++j) {
+        h->cbg[j] = o->ion_c0[j];
+        if (pc->cytosol_pool) h->conet_i[j] = pc->cyto_leak_mult * o->ion_D[j];
+        for (int u = 0; u < h->nbc; ++u) {
+            h->cbinds[j * h->nbc + u] = -1;
+            if (u < nb) {
+                int b = o->binds[j * nb + u];
+                if (b < 0) continue;
+                h->ckon[k] = o->bound_kon[b];
+                h->ckoff[k] = o->bound_koff[b];
+                D[k] = o->bound_D[b];
+                h->cb_dyad[k] = 1;
+            } else {
+                int c = u - nb;
+                if (!pc->cyto_buffer_binds[c * ni + j]) continue;
+                h->ckon[k] = pc->cyto_buffer_kon[c * ni + j];
+                h->ckoff[k] = pc->cyto_buffer_koff[c * ni + j];
+                D[k] = pc->cyto_buffer_D[c * ni + j];
+                h->cb_dyad[k] = 0;
+            }
+            h->cbinds[j * h->nbc + u] = (short)k;
+            if (pc->cytosol_pool) h->conet_b[k] = pc->cyto_leak_mult * D[k];
+            ++k;
+        }
+        if (pc->cytosol_pool && pc->serca_binds && pc->serca_binds[j]) {
+            h->serca[j] = 1;
+            h->jmax[j] = pc->serca_jmax[j];
+            h->kup[j] = pc->serca_kup[j];
+        }
+    }
+    h->nBc = k;
+    /* equilibrium of every cytosol buffer over the ions it binds (CytosolPool.cpp:42-70) */
+    for (int u = 0; u < h->nbc; ++u) {
+        double kon[16], koff[16], cc[16], sol[16];
+        int idx[16], m = 0;
+        for (int j = 0; j < ni; ++j) {
+            int b = h->cbinds[j * h->nbc + u];
+            if (b >= 0) {
+                kon[m] = h->ckon[b];
+                koff[m] = h->ckoff[b];
+                cc[m] = h->cbg[j];
+                idx[m++] = b;
+            }
+        }
+        if (m && o->start_eq) { /* the cytosol follows the dyad's "Start" (CytosolPool.cpp:49,60) */
+            oracle_buffer_equilibrium(m, kon, koff, cc, h->ctot[u], sol);
+            for (int t = 0; t < m; ++t) h->cinit[idx[t]] = sol[t];
+        }
+    }
+    h->cions = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    h->cevo_i = (double *)xcalloc((size_t)ni * S, sizeof(double));
+    h->cbuf = (double *)xcalloc((size_t)(h->nBc + 1) * S, sizeof(double));
+    h->cevo_b = (double *)xcalloc((size_t)(h->nBc + 1) * S, sizeof(double));
+    h->cfree = (double *)xcalloc((size_t)(h->nbc + 1) * S, sizeof(double));
+    h->cbb = (double *)xcalloc((size_t)(nB ? nB : 1) * S, sizeof(double));
+    return h;
+}
+
+void oracle_hybrid_destroy(OracleHybrid *h) {
+    if (!h) return;
+    void *ptrs[] = {h->nCQ, h->KCQ, h->tau, h->jc0, h->jions, h->jnew, h->jnsr, h->IjSR, h->nsr, h->cbinds,
+                    h->cb_dyad, h->ctot, h->ckon, h->ckoff, h->conet_b, h->cinit, h->conet_i, h->cbg, h->serca,
+                    h->jmax, h->kup, h->cions, h->cbuf, h->cfree, h->cevo_i, h->cevo_b, h->cbb, h->rim_ions,
+                    h->rim_bound};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+    free(h);
+}
+
+/* CytosolPool::_GetFreeBuffers (CytosolPool.cpp:270-288). */
+static void cyto_free(OracleHybrid *h, int s) {
+    int S = h->S;
+    for (int j = 0; j < h->ni; ++j) {
+        if (j == 0)
+            for (int u = 0; u < h->nbc; ++u) h->cfree[s + u * S] = h->ctot[u];
+        for (int u = 0; u < h->nbc; ++u) {
+            int b = h->cbinds[j * h->nbc + u];
+            if (b >= 0) h->cfree[s + u * S] -= h->cbuf[s + b * S];
+        }
+    }
+}
+
+/* ResetSystem (ParallelHybridSolver.cpp:149-157): NSR (no-op), SRPool::Reset
+ * (SRPool.cpp:103-116), dyad, CytosolPool::Reset (CytosolPool.cpp:290-315). */
+void oracle_hybrid_reset(OracleHybrid *h) {
+    int S = h->S, M = h->M;
+    for (int i = 0; i < h->ni; ++i) {
+        for (int s = 0; s < S; ++s) {
+            for (int c = 0; c < M; ++c) {
+                h->jions[c + M * (s + i * S)] = h->jc0[i];
+                h->jnew[c + M * (s + i * S)] = h->jc0[i];
+            }
+            h->IjSR[s + i * S] = 0;
+        }
+    }
+    oracle_reset(h->o);
+    for (int s = 0; s < S; ++s) {
+        int kd = 0;
+        for (int j = 0; j < h->ni; ++j) {
+            h->cions[s + j * S] = h->cbg[j];
+            for (int u = 0; u < h->nbc; ++u) {
+                if (j == 0) h->cfree[s + u * S] = h->ctot[u];
+                int b = h->cbinds[j * h->nbc + u];
+                if (b >= 0) {
+                    h->cbuf[s + b * S] = h->cinit[b];
+                    if (h->cb_dyad[b]) h->cbb[s + (kd++) * S] = h->cinit[b];
+                    h->cfree[s + u * S] -= h->cinit[b];
+                }
+            }
+        }
+    }
+}
+
+/* CytosolPool::RunRD (CytosolPool.cpp:217-268), effluxes from the last dyad update. */
+static void cyto_run(OracleHybrid *h, double dt, int s) {
+    int S = h->S, ni = h->ni, nIB = ni + h->nB;
+    const double *cur = h->o->gradients;
+    int kd = 0, ks = 0;
+    for (int j = 0; j < ni; ++j) {
+        double react = 0;
+        long str = s + j * S;
+        for (int u = 0; u < h->nbc; ++u) {
+            int b = h->cbinds[j * h->nbc + u];
+            if (b < 0) continue;
+            long s2 = s + b * S, s3 = s + u * S;
+            double R1 = h->ckoff[b] * h->cbuf[s2] - h->ckon[b] * h->cions[str] * h->cfree[s3];
+            h->cevo_b[s2] = -R1;
+            react += R1;
+            if (h->cb_dyad[b]) {
+                h->cevo_b[s2] += cur[ni + kd + s * nIB];
+                ++kd;
+            }
+        }
+        h->cevo_i[str] = react + cur[j + s * nIB];
+        if (h->serca[j]) {
+            /* ions_bg indexed by the SERCA counter, as CytosolPool.cpp:246 does */
+            double ions2 = h->cions[str] * h->cions[str];
+            double K2 = h->kup[j] * h->kup[j];
+            double b2 = h->cbg[ks] * h->cbg[ks];
+            h->cevo_i[str] -= h->jmax[j] * (ions2 / (ions2 + K2) - b2 / (b2 + K2));
+            ++ks;
+        }
+    }
+    for (int j = 0; j < ni; ++j) {
+        long str = s + j * S;
+        h->cevo_i[str] -= h->conet_i[j] * (h->cions[str] - h->cbg[j]);
+        h->cions[str] += h->cevo_i[str] * dt;
+    }
+    kd = 0;
+    for (int b = 0; b < h->nBc; ++b) {
+        long str = s + b * S;
+        h->cevo_b[str] -= h->conet_b[b] * (h->cbuf[str] - h->cinit[b]);
+        h->cbuf[str] += h->cevo_b[str] * dt;
+        if (h->cb_dyad[b]) h->cbb[s + (kd++) * S] = h->cbuf[str];
+    }
+    cyto_free(h, s);
+}
+
+/* SRPool::Run (SRPool.cpp:70-89). */
+static void jsr_run(OracleHybrid *h, double dt, int s) {
+    if (h->jsr_const) return;
+    int S = h->S, M = h->M, ni = h->ni;
+    double beta = 1;
+    for (int j = 0; j < ni; ++j) {
+        double c = h->ref_indexing ? h->jions[s + j * S] : h->jions[0 + M * (s + j * S)];
+        beta += h->nCQ[j] * h->KCQ[j] * h->cq / ((h->KCQ[j] + c) * (h->KCQ[j] + c));
+    }
+    for (int i = 0; i < ni; ++i) {
+        long id = 0 + M * (s + i * S);
+        double ns = h->ref_indexing ? h->jnsr[i + s * ni] : h->jnsr[s + i * S];
+        double Iref = 1 / h->tau[i] * (ns - h->jions[id]);
+        double newc = h->jions[id] + dt / beta * (Iref - h->IjSR[s + i * S] / h->Vjsr);
+        for (int c = 0; c < M; ++c) h->jnew[c + M * (s + i * S)] = newc;
+    }
+}
+
+/* ParallelHybridSolver::Update (ParallelHybridSolver.cpp:159-175). */
+void oracle_hybrid_update(OracleHybrid *h) {
+    OracleDyad *o = h->o;
+    if (h->cyto)
+        oracle_update(o, h->cions, h->cbb, h->jnew, o->V);
+    else
+        oracle_update(o, h->rim_ions, h->rim_bound, h->jnew, o->V);
+    /* SRPool::Update (SRPool.cpp:55-68) */
+    if (!h->jsr_const) {
+        int S = h->S, M = h->M;
+        for (int i = 0; i < h->ni; ++i)
+            for (int s = 0; s < S; ++s) {
+                h->IjSR[s + i * S] = o->total_sr_current[s + i * S] * h->Velem;
+                h->jnsr[s + i * S] = h->nsr[s + i * S];
+                for (int c = 0; c < M; ++c) h->jions[c + M * (s + i * S)] = h->jnew[c + M * (s + i * S)];
+            }
+    }
+}
+
+/* The loop body of RunSimulationsCPU (ParallelHybridSolver.cpp:420-430), k times. */
+void oracle_hybrid_step(OracleHybrid *h, double dt, int k, int n_threads) {
+    for (int it = 0; it < k; ++it) {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+        for (int s = 0; s < h->S; ++s) {
+            if (h->cyto) cyto_run(h, dt, s);
+            oracle_run_rd(h->o, dt, s);
+            jsr_run(h, dt, s);
+            oracle_run_mc(h->o, dt, s);
+        }
+        oracle_hybrid_update(h);
+    }
+}
+
+/* jSR concentration of every site and ion, [site + ion*S] (the value all of a site's channels see). */
+void oracle_hybrid_get_jsr(const OracleHybrid *h, double *out) {
+    for (int i = 0; i < h->ni; ++i)
+        for (int s = 0; s < h->S; ++s) out[s + i * h->S] = h->jnew[0 + h->M * (s + i * h->S)];
+}
+
+/* Cytosol ions [site + ion*S] and bound fields [site + b*S] (b over the cytosol's bound fields). */
+void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *bound) {
+    memcpy(ions, h->cions, sizeof(double) * h->ni * h->S);
+    if (bound) memcpy(bound, h->cbuf, sizeof(double) * h->nBc * h->S);
+}
+
+int oracle_hybrid_cyto_bound(const OracleHybrid *h) { return h->nBc; }

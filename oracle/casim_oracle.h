@@ -76,6 +76,22 @@ void oracle_get_total_sr_current(const OracleDyad *o, double *out);
 void oracle_get_rng_states(const OracleDyad *o, uint64_t *out);
 void oracle_set_rng_states(OracleDyad *o, const uint64_t *in);
 
+/* Compartments around the dyad and the reference's batch loop:
+ * SRPool, NSRConstPool, CytosolPool (or a constant rim) and
+ * ParallelHybridSolver::Update / RunSimulationsCPU.  ref_indexing = 1 makes
+ * SRPool::Run read exactly the elements the reference reads (SRPool.cpp:75,80),
+ * which equal the per-site elements only for one site. */
+typedef struct OracleHybrid OracleHybrid;
+OracleHybrid *oracle_hybrid_create(OracleDyad *o, const CasimPoolConfig *pc, const double *rim_ions,
+                                   const double *rim_bound, int ref_indexing);
+void oracle_hybrid_destroy(OracleHybrid *h);
+void oracle_hybrid_reset(OracleHybrid *h);
+void oracle_hybrid_update(OracleHybrid *h);
+void oracle_hybrid_step(OracleHybrid *h, double dt, int k, int n_threads);
+void oracle_hybrid_get_jsr(const OracleHybrid *h, double *out);
+void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *bound);
+int oracle_hybrid_cyto_bound(const OracleHybrid *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,14 @@ def load():
             "oracle_get_total_sr_current": (None, [P, _dp]),
             "oracle_get_rng_states": (None, [P, _up]),
             "oracle_set_rng_states": (None, [P, _up]),
+            "oracle_hybrid_create": (P, [P, C.c_void_p, _dp, _dp, C.c_int]),
+            "oracle_hybrid_destroy": (None, [P]),
+            "oracle_hybrid_reset": (None, [P]),
+            "oracle_hybrid_update": (None, [P]),
+            "oracle_hybrid_step": (None, [P, C.c_double, C.c_int, C.c_int]),
+            "oracle_hybrid_get_jsr": (None, [P, _dp]),
+            "oracle_hybrid_get_cytosol": (None, [P, _dp, _dp]),
+            "oracle_hybrid_cyto_bound": (C.c_int, [P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(lib, n)
@@ -221,3 +229,48 @@ def channel_rate(model, params, state, ca, mg=0.0):
     p = np.zeros(16)
     p[:len(params)] = params
     return load().oracle_channel_rate(model, d(p), state, ca, mg)
+
+
+class OracleHybridRun:
+    """The reference's batch loop around the oracle dyad: jSR pool, NSR and
+    cytosol pool (or a constant rim), RunSimulationsCPU's step order."""
+
+    def __init__(self, spec, pools, rim=None, ref_indexing=False):
+        self.dyad = Oracle(spec)
+        self.lib = self.dyad.lib
+        self.spec = spec
+        self._pc, self._pkeep = pools.to_struct(spec.ions)
+        ri, rb = rim if rim is not None else spec.rim_arrays()
+        self._ri = np.ascontiguousarray(ri, dtype=np.float64)
+        self._rb = np.ascontiguousarray(rb if len(rb) else np.zeros(1), dtype=np.float64)
+        self.h = self.lib.oracle_hybrid_create(self.dyad.h, C.byref(self._pc), d(self._ri), d(self._rb),
+                                               int(ref_indexing))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_hybrid_destroy(self.h)
+            self.h = None
+
+    def reset(self):
+        self.lib.oracle_hybrid_reset(self.h)
+
+    def init_opening(self, channels=None):
+        self.dyad.init_opening(channels)
+
+    def update(self):
+        self.lib.oracle_hybrid_update(self.h)
+
+    def step(self, dt, k, threads=None):
+        self.lib.oracle_hybrid_step(self.h, dt, k, threads or os.cpu_count() or 1)
+
+    def jsr(self):
+        out = np.empty(self.spec.n_ions * self.spec.n_sites)
+        self.lib.oracle_hybrid_get_jsr(self.h, d(out))
+        return out.reshape(self.spec.n_ions, self.spec.n_sites)
+
+    def cytosol(self):
+        nb = self.lib.oracle_hybrid_cyto_bound(self.h)
+        ions = np.empty(self.spec.n_ions * self.spec.n_sites)
+        bound = np.empty(max(nb, 1) * self.spec.n_sites)
+        self.lib.oracle_hybrid_get_cytosol(self.h, d(ions), d(bound))
+        return ions.reshape(self.spec.n_ions, -1), bound[:nb * self.spec.n_sites].reshape(nb, -1)

@@ -245,3 +245,24 @@ def test_sweep_2d_slots(gpu, oracle_mod):
     dev.Step(2e-6, 200)  # D dt (2/h^2 + 2/h^2) = 0.44: inside the Euler limit
     ora.step(2e-6, 200)
     compare(ora, dev, sp, FP64_TOL)
+
+
+def test_release_stats_match_oracle_trajectory(gpu, oracle_mod):
+    """On-device release statistics equal the IndChannelCREAnalyzer columns
+    accumulated from the oracle's channel states, step by step."""
+    from casim_amd.analysis import StatsTracker
+    sp = workloads.single_dyad(12, clamped=False)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    sp.channel_types[0].params["kclose0"] *= 1e3
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt = 1e-5
+    tr = StatsTracker(ora.channel_states(), dt)
+    for _ in range(200):
+        ora.step(dt, 1, threads=4)
+        tr.step(ora.channel_states())
+    dev.Step(dt, 200)
+    got = dev.GetReleaseStats()
+    want = tr.table()
+    assert np.array_equal(got[:, [0, 5]], want[:, [0, 5]])
+    assert np.allclose(got[:, 1:5], want[:, 1:5], rtol=1e-12, atol=1e-15)
+    assert want[:, 0].max() > 1
