@@ -85,6 +85,9 @@ def _declare(lib):
         "casim_dyad_reset": (C.c_int, [P]),
         "casim_dyad_init_opening": (C.c_int, [P, C.c_int32, C.c_int32, _ip]),
         "casim_dyad_update": (C.c_int, [P, _dp, _dp, _dp, C.c_double]),
+        "casim_dyad_attach_pools": (C.c_int, [P, C.POINTER(CasimPoolConfig)]),
+        "casim_dyad_get_pools": (C.c_int, [P, _dp, _dp, _dp]),
+        "casim_dyad_cytosol_bound": (C.c_int32, [P]),
         "casim_dyad_run_rd": (C.c_int, [P, C.c_double, C.c_int32]),
         "casim_dyad_run_mc": (C.c_int, [P, C.c_double, C.c_int32]),
         "casim_dyad_step": (C.c_int, [P, C.c_double, C.c_int32]),

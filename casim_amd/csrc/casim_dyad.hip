@@ -135,10 +135,21 @@ void CasimDyad::launch_step(double dt) {
 void CasimDyad::launch_channels(int mode, double dt) {
     if (precision == CASIM_FP64)
         hipLaunchKernelGGL((k_channels<double>), dim3(P.S), dim3(64), 0, stream, (const double *)g[cur],
-                           (const double *)f[cur], Ch, P, mode, dt);
+                           (const double *)f[cur], Ch, P, Q, mode, dt);
     else
         hipLaunchKernelGGL((k_channels<float>), dim3(P.S), dim3(64), 0, stream, (const float *)g[cur],
-                           (const float *)f[cur], Ch, P, mode, dt);
+                           (const float *)f[cur], Ch, P, Q, mode, dt);
+    HIPCK(hipGetLastError());
+}
+
+void CasimDyad::launch_rim(const double *ions, const double *bound) {
+    dim3 grid((P.NS + 255) / 256, P.S);
+    if (precision == CASIM_FP64)
+        hipLaunchKernelGGL(k_rim<double>, grid, dim3(256), 0, stream, (double *)g[0], (double *)g[1],
+                           (double *)f[0], (double *)f[1], ions, bound, P);
+    else
+        hipLaunchKernelGGL(k_rim<float>, grid, dim3(256), 0, stream, (float *)g[0], (float *)g[1], (float *)f[0],
+                           (float *)f[1], ions, bound, P);
     HIPCK(hipGetLastError());
 }
 
@@ -305,6 +316,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         d->sym = c->dim == 3 ? c->dx * c->dz / c->dy : c->dx * c->z / c->dy;
         d->precision = c->precision;
         d->tsize = c->precision == CASIM_FP64 ? 8 : 4;
+        d->start_equilibrium = c->start_equilibrium != 0;
         {
             // plane-marching sweep: one plane of every grid field in LDS (x2)
             const int Pl = P.ny * P.nz;
@@ -464,6 +476,8 @@ int casim_dyad_destroy(CasimDyad *d) {
                     d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (double *p : d->pool_buffers)
+        if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
     return CASIM_OK;
@@ -485,6 +499,11 @@ int casim_dyad_reset(CasimDyad *d) {
         HIPCK(hipGetLastError());
         HIPCK(hipMemsetAsync(d->d_state, 0, sizeof(int) * std::max(1, d->nch), d->stream));
         HIPCK(hipMemsetAsync(d->d_stats, 0, sizeof(double) * NSTAT * P.S, d->stream));
+        if (d->Q.on) {
+            hipLaunchKernelGGL(k_pools_reset, dim3((P.S + 127) / 128), dim3(128), 0, d->stream, d->Q, P.S, P.ni,
+                               d->d_jc0);
+            HIPCK(hipGetLastError());
+        }
         HIPCK(hipMemcpyAsync(d->d_rng, d->d_rng0, sizeof(uint32_t) * P.S * 6, hipMemcpyDeviceToDevice, d->stream));
         HIPCK(hipStreamSynchronize(d->stream));
         d->cur = 0;
@@ -521,30 +540,177 @@ int casim_dyad_init_opening(CasimDyad *d, int32_t site, int32_t channel, const i
 int casim_dyad_update(CasimDyad *d, const double *rim_ions, const double *rim_bound, const double *jsr,
                       double voltage) {
     return guard([&] {
-        REQUIRE(d && rim_ions && jsr && (rim_bound || d->P.nb == 0), CASIM_E_INVALID, "null argument");
+        REQUIRE(d, CASIM_E_INVALID, "null handle");
+        const bool pool_rim = d->Q.on && d->Q.cyto;
+        REQUIRE(pool_rim || (rim_ions && (rim_bound || d->P.nb == 0)), CASIM_E_INVALID, "null rim argument");
+        REQUIRE(d->Q.on || jsr, CASIM_E_INVALID, "null jsr argument");
         d->use();
         const Params &P = d->P;
         d->voltage = voltage;
-        HIPCK(hipMemcpyAsync(d->d_rim_ions, rim_ions, sizeof(double) * P.ni * P.S, hipMemcpyHostToDevice, d->stream));
-        if (P.nb)
-            HIPCK(hipMemcpyAsync(d->d_rim_bound, rim_bound, sizeof(double) * P.nb * P.S, hipMemcpyHostToDevice,
+        if (!pool_rim) {
+            HIPCK(hipMemcpyAsync(d->d_rim_ions, rim_ions, sizeof(double) * P.ni * P.S, hipMemcpyHostToDevice,
                                  d->stream));
-        if (d->M)
+            if (P.nb)
+                HIPCK(hipMemcpyAsync(d->d_rim_bound, rim_bound, sizeof(double) * P.nb * P.S, hipMemcpyHostToDevice,
+                                     d->stream));
+        }
+        if (!d->Q.on && d->M)
             HIPCK(hipMemcpyAsync(d->d_jsr, jsr, sizeof(double) * P.ni * P.S * d->M, hipMemcpyHostToDevice, d->stream));
-        dim3 grid((P.NS + 255) / 256, P.S);
-        if (d->precision == CASIM_FP64)
-            hipLaunchKernelGGL(k_rim<double>, grid, dim3(256), 0, d->stream, (double *)d->g[0], (double *)d->g[1],
-                               (double *)d->f[0], (double *)d->f[1], d->d_rim_ions, d->d_rim_bound, P);
+        if (pool_rim)
+            d->launch_rim(d->Q.cions, d->Q.cbb);
         else
-            hipLaunchKernelGGL(k_rim<float>, grid, dim3(256), 0, d->stream, (float *)d->g[0], (float *)d->g[1],
-                               (float *)d->f[0], (float *)d->f[1], d->d_rim_ions, d->d_rim_bound, P);
-        HIPCK(hipGetLastError());
-        d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE, 0.0);
+            d->launch_rim(d->d_rim_ions, d->d_rim_bound);
         d->launch_efflux();
+        d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, 0.0);
+        d->clear_graphs();
         HIPCK(hipStreamSynchronize(d->stream));
         d->have_update = true;
     });
 }
+
+}  // extern "C"
+
+namespace {
+void copy_out(CasimDyad *d, double *out, const double *src, size_t n) {
+    HIPCK(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+    HIPCK(hipStreamSynchronize(d->stream));
+}
+}  // namespace
+
+extern "C" {
+
+int casim_dyad_attach_pools(CasimDyad *d, const CasimPoolConfig *pc) {
+    return guard([&] {
+        REQUIRE(d && pc, CASIM_E_INVALID, "null argument");
+        REQUIRE(!d->Q.on, CASIM_E_STATE, "pools are already attached");
+        d->use();
+        const Params &P = d->P;
+        Pools Q{};
+        const int ni = P.ni, S = P.S;
+        Q.jsr_const = pc->jsr_constant;
+        Q.cq = pc->jsr_cq_total;
+        Q.Vjsr = pc->jsr_volume;
+        REQUIRE(pc->jsr_constant || pc->jsr_volume > 0, CASIM_E_INVALID, "jSR volume must be > 0");
+        Q.Velem = d->elem_volume;
+        std::vector<double> jc0(ni), nsr((size_t)ni * S), jnsr((size_t)ni * S);
+        for (int i = 0; i < ni; ++i) {
+            Q.nCQ[i] = pc->jsr_cq_sites[i];
+            Q.KCQ[i] = pc->jsr_k_cq[i];
+            Q.tau[i] = pc->jsr_tau_refill[i];
+            REQUIRE(pc->jsr_constant || Q.tau[i] != 0.0, CASIM_E_INVALID, "jSR refill time must be non-zero");
+            jc0[i] = pc->jsr_c0[i];
+            for (int s = 0; s < S; ++s) {
+                nsr[s + (size_t)i * S] = pc->nsr_c[i];
+                jnsr[s + (size_t)i * S] = jc0[i];  // SRPool's ctor (SRPool.cpp:67-72)
+            }
+        }
+        Q.cyto = pc->cytosol_pool;
+        const int nc = pc->cytosol_pool ? pc->n_cyto_buffers : 0;
+        REQUIRE(nc >= 0 && P.nu + nc <= MAXCU, CASIM_E_INVALID, "too many cytosol buffers");
+        Q.nbc = P.nu + nc;
+        int k = 0;
+        double Dk[MAXCB];
+        for (int j = 0; j < ni; ++j) {
+            for (int u = 0; u < Q.nbc; ++u) {
+                Q.cbind[j * Q.nbc + u] = -1;
+                double kon, koff, D;
+                int dy;
+                if (u < P.nu) {
+                    int b = -1;
+                    for (int bb = 0; bb < P.nb; ++bb)
+                        if (P.bion[bb] == j && P.bbuf[bb] == u) b = bb;
+                    if (b < 0) continue;
+                    kon = P.kon[b];
+                    koff = P.koff[b];
+                    D = P.Db[b];
+                    dy = 1;
+                } else {
+                    const int c = u - P.nu;
+                    if (!pc->cyto_buffer_binds[c * ni + j]) continue;
+                    kon = pc->cyto_buffer_kon[c * ni + j];
+                    koff = pc->cyto_buffer_koff[c * ni + j];
+                    D = pc->cyto_buffer_D[c * ni + j];
+                    dy = 0;
+                }
+                REQUIRE(k < MAXCB, CASIM_E_INVALID, "too many cytosol bound fields");
+                Q.cbind[j * Q.nbc + u] = k;
+                Q.ckon[k] = kon;
+                Q.ckoff[k] = koff;
+                Q.cb_dyad[k] = dy;
+                Dk[k] = D;
+                Q.conet_b[k] = pc->cytosol_pool ? pc->cyto_leak_mult * D : 0.0;
+                ++k;
+            }
+            Q.cbg[j] = P.c0[j];
+            Q.conet_i[j] = pc->cytosol_pool ? pc->cyto_leak_mult * P.Dion[j] : 0.0;
+            Q.serca[j] = (pc->cytosol_pool && pc->serca_binds) ? pc->serca_binds[j] : 0;
+            Q.jmax[j] = Q.serca[j] ? pc->serca_jmax[j] : 0.0;
+            Q.kup[j] = Q.serca[j] ? pc->serca_kup[j] : 0.0;
+        }
+        (void)Dk;
+        Q.nBc = k;
+        for (int u = 0; u < Q.nbc; ++u) Q.ctot[u] = u < P.nu ? P.total[u] : pc->cyto_buffer_total[u - P.nu];
+        if (d->start_equilibrium) {
+            for (int u = 0; u < Q.nbc; ++u) {
+                std::vector<double> kon, koff, cc;
+                std::vector<int> idx;
+                for (int j = 0; j < ni; ++j) {
+                    const int b = Q.cbind[j * Q.nbc + u];
+                    if (b < 0) continue;
+                    kon.push_back(Q.ckon[b]);
+                    koff.push_back(Q.ckoff[b]);
+                    cc.push_back(Q.cbg[j]);
+                    idx.push_back(b);
+                }
+                if (idx.empty()) continue;
+                auto sol = buffer_equilibrium(kon, koff, cc, Q.ctot[u]);
+                for (size_t t = 0; t < idx.size(); ++t) Q.cinit[idx[t]] = sol[t];
+            }
+        }
+        auto buf = [&](size_t n) {
+            double *p = dalloc<double>(n);
+            d->pool_buffers.push_back(p);
+            HIPCK(hipMemsetAsync(p, 0, sizeof(double) * std::max<size_t>(n, 1), d->stream));
+            return p;
+        };
+        Q.jions = buf((size_t)ni * S);
+        Q.jnew = buf((size_t)ni * S);
+        Q.IjSR = buf((size_t)ni * S);
+        Q.jnsr = buf((size_t)ni * S);
+        double *dn = buf((size_t)ni * S);
+        Q.nsr = dn;
+        Q.cions = buf((size_t)ni * S);
+        Q.cbuf = buf((size_t)std::max(1, Q.nBc) * S);
+        Q.cfree = buf((size_t)std::max(1, Q.nbc) * S);
+        Q.cbb = buf((size_t)std::max(1, P.nb) * S);
+        d->d_jc0 = buf(ni);
+        HIPCK(hipMemcpyAsync(dn, nsr.data(), sizeof(double) * nsr.size(), hipMemcpyHostToDevice, d->stream));
+        HIPCK(hipMemcpyAsync(Q.jnsr, jnsr.data(), sizeof(double) * jnsr.size(), hipMemcpyHostToDevice, d->stream));
+        HIPCK(hipMemcpyAsync(d->d_jc0, jc0.data(), sizeof(double) * ni, hipMemcpyHostToDevice, d->stream));
+        Q.eff = d->d_eff;
+        Q.on = 1;
+        d->Q = Q;
+        hipLaunchKernelGGL(k_pools_reset, dim3((S + 127) / 128), dim3(128), 0, d->stream, d->Q, S, ni, d->d_jc0);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(d->stream));
+        d->have_update = false;
+        d->clear_graphs();
+    });
+}
+
+int casim_dyad_get_pools(CasimDyad *d, double *jsr, double *cyto_ions, double *cyto_bound) {
+    return guard([&] {
+        REQUIRE(d, CASIM_E_INVALID, "null handle");
+        REQUIRE(d->Q.on, CASIM_E_STATE, "no pools attached");
+        d->use();
+        const size_t n = (size_t)d->P.ni * d->P.S;
+        if (jsr) copy_out(d, jsr, d->Q.jnew, n);
+        if (cyto_ions) copy_out(d, cyto_ions, d->Q.cions, n);
+        if (cyto_bound && d->Q.nBc) copy_out(d, cyto_bound, d->Q.cbuf, (size_t)d->Q.nBc * d->P.S);
+    });
+}
+
+int32_t casim_dyad_cytosol_bound(const CasimDyad *d) { return d && d->Q.on ? d->Q.nBc : 0; }
 
 int casim_dyad_run_rd(CasimDyad *d, double dt, int32_t site) {
     return guard([&] {
@@ -580,14 +746,22 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
         if (it == d->graphs.end()) {
             HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
             try {
+                const bool pool_rim = d->Q.on && d->Q.cyto;
                 for (int k = 0; k < k_steps; ++k) {
+                    if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux
+                        d->launch_rim(d->Q.cions, d->Q.cbb);
+                        d->launch_efflux();
+                    }
                     // step 1 recomputes the currents of the preceding update, so
                     // it does not add to the accumulated totals again
-                    d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0), dt);
+                    d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
+                                           CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                       dt);
                     d->launch_step(dt);
                 }
-                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE, dt);
+                if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
                 d->launch_efflux();
+                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, dt);
             } catch (...) {
                 hipGraph_t junk;
                 (void)hipStreamEndCapture(d->stream, &junk);
@@ -616,16 +790,24 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
         std::vector<hipEvent_t> ev(4 * (size_t)k_steps);
         for (auto &e : ev) HIPCK(hipEventCreate(&e));
         try {
+            const bool pool_rim = d->Q.on && d->Q.cyto;
             for (int k = 0; k < k_steps; ++k) {
+                if (pool_rim) {
+                    d->launch_rim(d->Q.cions, d->Q.cbb);
+                    d->launch_efflux();
+                }
                 HIPCK(hipEventRecord(ev[4 * k], d->stream));
-                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0), dt);
+                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) | CH_POOL_UPDATE |
+                                       CH_POOL_ADVANCE,
+                                   dt);
                 HIPCK(hipEventRecord(ev[4 * k + 1], d->stream));
                 HIPCK(hipEventRecord(ev[4 * k + 2], d->stream));
                 d->launch_step(dt);
                 HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
             }
-            d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE, dt);
+            if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
             d->launch_efflux();
+            d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, dt);
             HIPCK(hipStreamSynchronize(d->stream));
             double ms_ch = 0, ms_st = 0;
             for (int k = 0; k < k_steps; ++k) {
@@ -653,12 +835,7 @@ int casim_dyad_synchronize(CasimDyad *d) {
     });
 }
 
-namespace {
-void copy_out(CasimDyad *d, double *out, const double *src, size_t n) {
-    HIPCK(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
-    HIPCK(hipStreamSynchronize(d->stream));
-}
-}  // namespace
+
 
 int casim_dyad_get_effluxes(CasimDyad *d, double *out) {
     return guard([&] {

@@ -37,6 +37,9 @@ struct CasimDyad {
     hipStream_t stream = nullptr;
     casim::Params P{};
     casim::Channels Ch{};
+    casim::Pools Q{};           // compartments around the dyad (Q.on = 0 when not attached)
+    double *d_jc0 = nullptr;    // jSR start concentration per ion
+    std::vector<double *> pool_buffers;
     int precision = CASIM_FP64;
     size_t tsize = 8;
     void *g[2] = {nullptr, nullptr};
@@ -53,6 +56,7 @@ struct CasimDyad {
     double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
            *d_rim_bound = nullptr, *d_eff = nullptr, *d_stats = nullptr;
     bool have_update = false;
+    bool start_equilibrium = false;
     int threads = 256;        // k_step block size
     int sweep_threads = 0;    // k_sweep block size (0: not usable)
     int sweep_chunk = 0;      // interior x-planes per k_sweep workgroup
@@ -68,6 +72,7 @@ struct CasimDyad {
     void launch_step(double dt);
     void launch_channels(int mode, double dt);
     void launch_efflux();
+    void launch_rim(const double *ions, const double *bound);
     void clear_graphs() {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();

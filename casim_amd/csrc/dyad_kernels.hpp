@@ -64,7 +64,116 @@ struct Channels {
     double g[MAXT * MAXI];
 };
 
-enum ChannelMode { CH_SNAPSHOT = 1, CH_CURRENTS = 2, CH_GATE = 4, CH_GATE_STORED = 8, CH_ACCUMULATE = 16 };
+enum ChannelMode {
+    CH_SNAPSHOT = 1,
+    CH_CURRENTS = 2,
+    CH_GATE = 4,
+    CH_GATE_STORED = 8,
+    CH_ACCUMULATE = 16,
+    CH_POOL_UPDATE = 32,   // SRPool::Update from the fresh site totals
+    CH_POOL_ADVANCE = 64,  // CytosolPool::RunRD and SRPool::Run for the next step
+};
+
+constexpr int MAXCU = 12;  // cytosol buffers (dyad buffers first)
+constexpr int MAXCB = 16;  // cytosol bound fields
+
+// Compartments around the dyad held on the device (SRPool.cpp,
+// NSRConstPool.cpp, CytosolPool.cpp), one value per site and species.
+struct Pools {
+    int on;           // pools attached
+    int jsr_const, cyto;
+    double cq, Vjsr, Velem;
+    double nCQ[MAXI], KCQ[MAXI], tau[MAXI];
+    double *jions, *jnew, *IjSR, *jnsr;  // [site + ion*S]
+    const double *nsr;                   // [site + ion*S]
+    int nbc, nBc;
+    int cbind[MAXI * MAXCU];             // [ion * nbc + buffer] -> bound index or -1
+    int cb_dyad[MAXCB];
+    double ctot[MAXCU], ckon[MAXCB], ckoff[MAXCB], conet_b[MAXCB], cinit[MAXCB];
+    double conet_i[MAXI], cbg[MAXI];
+    int serca[MAXI];
+    double jmax[MAXI], kup[MAXI];
+    double *cions, *cbuf, *cfree, *cbb;  // [site + k*S]
+    const double *eff;                   // dyad effluxes [f + (ni+nb)*site]
+};
+
+// SRPool::Update (SRPool.cpp:55-68) for one site.
+static __device__ void pool_update_site(const Pools &q, int s, int S, int ni, const double *totals) {
+#pragma clang fp contract(off)
+    if (q.jsr_const) return;
+    for (int i = 0; i < ni; ++i) {
+        q.IjSR[s + i * S] = totals[s + i * S] * q.Velem;
+        q.jnsr[s + i * S] = q.nsr[s + i * S];
+        q.jions[s + i * S] = q.jnew[s + i * S];
+    }
+}
+
+// CytosolPool::RunRD (CytosolPool.cpp:217-268) and SRPool::Run
+// (SRPool.cpp:70-89) for one site, in the reference's operation order.
+static __device__ void pool_advance_site(const Pools &q, int s, int S, int ni, int nb, double dt) {
+#pragma clang fp contract(off)
+    if (q.cyto) {
+        const int nIB = ni + nb;
+        double evo_i[MAXI], evo_b[MAXCB];
+        int kd = 0, ks = 0;
+        for (int j = 0; j < ni; ++j) {
+            double react = 0;
+            for (int u = 0; u < q.nbc; ++u) {
+                const int b = q.cbind[j * q.nbc + u];
+                if (b < 0) continue;
+                const double R1 = q.ckoff[b] * q.cbuf[s + b * S] - q.ckon[b] * q.cions[s + j * S] * q.cfree[s + u * S];
+                evo_b[b] = -R1;
+                react += R1;
+                if (q.cb_dyad[b]) {
+                    evo_b[b] += q.eff[ni + kd + s * nIB];
+                    ++kd;
+                }
+            }
+            evo_i[j] = react + q.eff[j + s * nIB];
+            if (q.serca[j]) {
+                const double c = q.cions[s + j * S];
+                const double ions2 = c * c;
+                const double K2 = q.kup[j] * q.kup[j];
+                const double b2 = q.cbg[ks] * q.cbg[ks];  // indexed by the SERCA counter, as the reference
+                evo_i[j] -= q.jmax[j] * (ions2 / (ions2 + K2) - b2 / (b2 + K2));
+                ++ks;
+            }
+        }
+        for (int j = 0; j < ni; ++j) {
+            const double c = q.cions[s + j * S];
+            evo_i[j] -= q.conet_i[j] * (c - q.cbg[j]);
+            q.cions[s + j * S] = c + evo_i[j] * dt;
+        }
+        kd = 0;
+        for (int b = 0; b < q.nBc; ++b) {
+            const double c = q.cbuf[s + b * S];
+            evo_b[b] -= q.conet_b[b] * (c - q.cinit[b]);
+            const double nv = c + evo_b[b] * dt;
+            q.cbuf[s + b * S] = nv;
+            if (q.cb_dyad[b]) q.cbb[s + (kd++) * S] = nv;
+        }
+        for (int j = 0; j < ni; ++j) {
+            if (j == 0)
+                for (int u = 0; u < q.nbc; ++u) q.cfree[s + u * S] = q.ctot[u];
+            for (int u = 0; u < q.nbc; ++u) {
+                const int b = q.cbind[j * q.nbc + u];
+                if (b >= 0) q.cfree[s + u * S] -= q.cbuf[s + b * S];
+            }
+        }
+    }
+    if (!q.jsr_const) {
+        double beta = 1;
+        for (int j = 0; j < ni; ++j) {
+            const double c = q.jions[s + j * S];
+            beta += q.nCQ[j] * q.KCQ[j] * q.cq / ((q.KCQ[j] + c) * (q.KCQ[j] + c));
+        }
+        for (int i = 0; i < ni; ++i) {
+            const double c = q.jions[s + i * S];
+            const double Iref = 1 / q.tau[i] * (q.jnsr[s + i * S] - c);
+            q.jnew[s + i * S] = c + dt / beta * (Iref - q.IjSR[s + i * S] / q.Vjsr);
+        }
+    }
+}
 
 // Per-site release statistics, the columns of IndChannelCREAnalyzer's
 // scenarios table (IndChannelCREAnalyzer.cpp:89-137) accumulated step by step:
@@ -563,7 +672,7 @@ __device__ __forceinline__ double draw_at(const uint32_t *__restrict__ jump, con
 // re-evaluated with the extra draw accounted for, until a ballot is empty.
 template <typename T>
 __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const T *__restrict__ f, Channels a,
-                                                 Params p, int mode, double dt) {
+                                                 Params p, Pools q, int mode, double dt) {
     __shared__ double Jl[MAXI][64];
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
@@ -623,7 +732,8 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
             for (int k = 0; k < MAXI; ++k) {
                 double J = 0.0;
                 if (k < p.ni && valid) {
-                    double cond = a.g[typ * MAXI + k] * (a.jsr[i + (size_t)a.M * (s + (size_t)k * p.S)] - ion[k]);
+                    const double jv = q.on ? q.jnew[s + k * p.S] : a.jsr[i + (size_t)a.M * (s + (size_t)k * p.S)];
+                    double cond = a.g[typ * MAXI + k] * (jv - ion[k]);
                     J = (double)st * cond;
                     a.sJ[(size_t)k * a.nch + off + sp] = J;
                 }
@@ -671,6 +781,10 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
     if ((mode & CH_CURRENTS) && (mode & CH_ACCUMULATE) && lane == 0) {
         for (int k = 0; k < p.ni; ++k) a.totals[s + k * p.S] = acc[k];
     }
+    if (q.on && lane == 0) {
+        if (mode & CH_POOL_UPDATE) pool_update_site(q, s, p.S, p.ni, a.totals);
+        if (mode & CH_POOL_ADVANCE) pool_advance_site(q, s, p.S, p.ni, p.nb, dt);
+    }
     if ((mode & (CH_GATE | CH_GATE_STORED)) && lane == 0) {
         double *st = a.stats + (size_t)s * NSTAT;
         const double steps = st[ST_STEPS] + 1.0;
@@ -712,6 +826,32 @@ static __global__ void k_init_opening(Channels a, int S, int site, int channel, 
     mrg::matvec(J, S6, out, mrg::C1, mrg::M1);
     mrg::matvec(J + 9, S6 + 3, out + 3, mrg::C2, mrg::M2);
     for (int i = 0; i < 6; ++i) a.rng[s * 6 + i] = out[i];
+}
+
+// SRPool::Reset (SRPool.cpp:103-116) and CytosolPool::Reset
+// (CytosolPool.cpp:290-315); the jSR's NSR copy starts at the jSR value
+// (SRPool.cpp:67-72).  One thread per site.
+static __global__ void k_pools_reset(Pools q, int S, int ni, const double *__restrict__ jc0) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    for (int i = 0; i < ni; ++i) {
+        q.jions[s + i * S] = q.jnew[s + i * S] = jc0[i];
+        q.IjSR[s + i * S] = 0.0;
+    }
+    if (!q.cyto) return;
+    int kd = 0;
+    for (int j = 0; j < ni; ++j) {
+        q.cions[s + j * S] = q.cbg[j];
+        for (int u = 0; u < q.nbc; ++u) {
+            if (j == 0) q.cfree[s + u * S] = q.ctot[u];
+            const int b = q.cbind[j * q.nbc + u];
+            if (b >= 0) {
+                q.cbuf[s + b * S] = q.cinit[b];
+                if (q.cb_dyad[b]) q.cbb[s + (kd++) * S] = q.cinit[b];
+                q.cfree[s + u * S] -= q.cinit[b];
+            }
+        }
+    }
 }
 
 // Reset (DyadRDCPU.cpp:203-229) of both ping-pong copies.

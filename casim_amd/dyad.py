@@ -57,6 +57,29 @@ class DyadHIP:
         abi.check(self.lib.casim_dyad_update(self.h, abi.dptr(rim_ions), abi.dptr(rim_bound), abi.dptr(jsr),
                                              voltage))
 
+    def AttachPools(self, pools):
+        """jSR / NSR / cytosol compartments on the device (config.PoolSpec)."""
+        self._pc, self._pkeep = pools.to_struct(self.spec.ions)
+        abi.check(self.lib.casim_dyad_attach_pools(self.h, C.byref(self._pc)))
+
+    def UpdatePools(self, rim_ions=None, rim_bound=None, voltage=0.0):
+        """ParallelHybridSolver::Update with the pools attached."""
+        ri = None if rim_ions is None else np.ascontiguousarray(rim_ions, dtype=np.float64)
+        rb = None if rim_bound is None else np.ascontiguousarray(rim_bound, dtype=np.float64)
+        if rb is not None and rb.size == 0:
+            rb = np.zeros(1)
+        abi.check(self.lib.casim_dyad_update(self.h, None if ri is None else abi.dptr(ri),
+                                             None if rb is None else abi.dptr(rb), None, voltage))
+
+    def GetPools(self):
+        """(jSR [ion, site], cytosol ions [ion, site], cytosol bound [b, site])."""
+        n = self.spec.n_ions * self.S
+        nb = self.lib.casim_dyad_cytosol_bound(self.h)
+        j, ci, cb = np.empty(n), np.empty(n), np.empty(max(nb, 1) * self.S)
+        abi.check(self.lib.casim_dyad_get_pools(self.h, abi.dptr(j), abi.dptr(ci), abi.dptr(cb)))
+        return (j.reshape(self.spec.n_ions, -1), ci.reshape(self.spec.n_ions, -1),
+                cb[:nb * self.S].reshape(nb, -1))
+
     def RunRD(self, dt, site=-1):
         abi.check(self.lib.casim_dyad_run_rd(self.h, dt, site))
 

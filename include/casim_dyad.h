@@ -197,6 +197,28 @@ int casim_dyad_init_opening(CasimDyad *d, int32_t site, int32_t channel, const i
 int casim_dyad_update(CasimDyad *d, const double *rim_ions, const double *rim_bound,
                       const double *jsr, double voltage);
 
+/*
+ * Attach the compartments around the dyad: the jSR pool (SRPool.cpp), the
+ * constant network SR (NSRConstPool.cpp) and, with cytosol_pool = 1, the
+ * well-mixed cytosol (CytosolPool.cpp), all held on the device per site.
+ * From then on casim_dyad_update takes its jSR from the pool (jsr may be
+ * NULL) and, with a cytosol pool, its rim too (rim arrays may be NULL), and
+ * casim_dyad_step advances the pools with the dyad in the reference's step
+ * order (ParallelHybridSolver.cpp:420-430: cytosol RunRD, dyad RunRD, jSR
+ * Run, gating, then Update).  casim_dyad_reset resets the pools with the
+ * dyad (ResetSystem, ParallelHybridSolver.cpp:149-157).  The protocol calls
+ * run_rd / run_mc do not advance the pools.  Call once per handle.
+ */
+int casim_dyad_attach_pools(CasimDyad *d, const CasimPoolConfig *pc);
+
+/* Pool state: jSR [site + ion*S] (the value every channel of a site sees,
+ * SRPool::GetIons), cytosol ions [site + ion*S] and cytosol bound fields
+ * [site + b*S], b over the cytosol's bound fields (ion-major over dyad
+ * buffers then cytosol buffers; count from casim_dyad_cytosol_bound).
+ * Any output may be NULL. */
+int casim_dyad_get_pools(CasimDyad *d, double *jsr, double *cyto_ions, double *cyto_bound);
+int32_t casim_dyad_cytosol_bound(const CasimDyad *d);
+
 /* DyadSeries::RunRD -> DyadRDCPU::RunRD (DyadRDCPU.cpp:495-506) for every
  * site at once (the reference GPU form, DyadRDMP.cu:853-860).  site must be -1. */
 int casim_dyad_run_rd(CasimDyad *d, double dt, int32_t site);

@@ -266,3 +266,57 @@ def test_release_stats_match_oracle_trajectory(gpu, oracle_mod):
     assert np.array_equal(got[:, [0, 5]], want[:, [0, 5]])
     assert np.allclose(got[:, 1:5], want[:, 1:5], rtol=1e-12, atol=1e-15)
     assert want[:, 0].max() > 1
+
+
+def _pool_pair(oracle_mod, sp, ps, rim=None):
+    ora = oracle_mod.OracleHybridRun(sp, ps, rim=rim)
+    ora.reset()
+    ora.init_opening()
+    ora.update()
+    dev = DyadHIP(sp)
+    dev.AttachPools(ps)
+    dev.InitOpeningAll()
+    ri, rb = rim if rim is not None else sp.rim_arrays()
+    dev.UpdatePools(None if ps.cytosol_pool else ri, None if ps.cytosol_pool else rb)
+    return ora, dev
+
+
+def test_pools_jsr_depletion_fp64(gpu, oracle_mod):
+    """Depleting jSR (calsequestrin-buffered SRPool) with a constant rim."""
+    from casim_amd.config import PoolSpec
+    sp = workloads.single_dyad(4, clamped=False)
+    sp.channel_types[0].params["kopen0"] *= 1e4
+    ps = PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=2e5, jsr_c0=[1000.0], jsr_cq_sites=[1.0],
+                  jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])
+    ora, dev = _pool_pair(oracle_mod, sp, ps)
+    dt = 1e-5
+    for _ in range(4):
+        dev.Step(dt, 150)
+        ora.step(dt, 150)
+        assert np.array_equal(dev.GetChannelsStates(), ora.dyad.channel_states())
+    j_dev = dev.GetPools()[0]
+    assert rel_err(j_dev, ora.jsr()) < FP64_TOL
+    assert ora.jsr().min() < 999.0
+    compare(ora.dyad, dev, sp, FP64_TOL)
+    assert np.array_equal(dev.GetRngStates(), ora.dyad.rng_states())
+
+
+def test_pools_reference_test_input_cytosol(gpu, oracle_mod):
+    """The reference's test configuration end to end: CytosolPool (leak,
+    SERCA, troponin), SRPool with calsequestrin, constant NSR, 2D dyad."""
+    sp, j = config.load_json(os.path.join(GOLDEN, "test_input_Parameters.json"), n_sites=3)
+    sp.init_channel = [2, 9, 16]
+    ps = config.pools_from_json(j, sp)
+    ora, dev = _pool_pair(oracle_mod, sp, ps)
+    dt = float(j["dt, ns"]) * 1e-6
+    for _ in range(3):
+        dev.Step(dt, 400)
+        ora.step(dt, 400)
+        assert np.array_equal(dev.GetChannelsStates(), ora.dyad.channel_states())
+    jd, cid, cbd = dev.GetPools()
+    cio, cbo = ora.cytosol()
+    assert rel_err(jd, ora.jsr()) < FP64_TOL
+    assert rel_err(cid, cio) < FP64_TOL
+    assert rel_err(cbd, cbo) < FP64_TOL
+    assert ora.jsr()[0].min() < 1000.0 and cio[0].max() > 0.1
+    compare(ora.dyad, dev, sp, FP64_TOL)
