@@ -109,14 +109,14 @@ def cpu_baseline(workloads, n_sites=32, steps=300):
 def traffic_from_profiles():
     """HBM bytes per sweep-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, KiB, scaled by the copy-kernel factor)."""
-    p = os.path.join(ROOT, "profiles", "k_step_fp64_batched4096_pmc.json")
+    p = os.path.join(ROOT, "profiles", "k_sweep_batched4096_81x81x5.md")
     if not os.path.exists(p):
         return None
-    try:
-        with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (ValueError, KeyError):
-        return None
+    with open(p) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    return None
 
 
 def main():

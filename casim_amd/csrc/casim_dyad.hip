@@ -22,24 +22,14 @@
 
 using namespace casim;
 
-namespace {
-
-thread_local std::string g_err;
-
-template <class F>
-int guard(F &&f) {
-    try {
-        f();
-        g_err.clear();
-        return CASIM_OK;
-    } catch (casim::Err &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (std::exception &e) {
-        g_err = e.what();
-        return CASIM_E_INVALID;
-    }
+namespace casim {
+std::string &last_error() {
+    thread_local std::string e;
+    return e;
 }
+}  // namespace casim
+
+namespace {
 
 // GaussElimination of DyadRDCPU.cpp:163-193 on the equilibrium system of one
 // buffer (:53-81): row t: sum_I kon_t C_t B_I + koff_t B_t = kon_t C_t Btot.
@@ -167,7 +157,7 @@ void CasimDyad::launch_efflux() {
 
 extern "C" {
 
-const char *casim_last_error(void) { return g_err.c_str(); }
+const char *casim_last_error(void) { return casim::last_error().c_str(); }
 
 const char *casim_version(void) { return "casim_dyad 0.1 gfx950"; }
 
@@ -455,13 +445,13 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         Ch.totals = d->d_totals;
         Ch.stats = d->d_stats;
         HIPCK(hipStreamSynchronize(d->stream));
-        if (casim_dyad_reset(d) != CASIM_OK) throw Err(CASIM_E_DEVICE, g_err);
+        if (casim_dyad_reset(d) != CASIM_OK) throw Err(CASIM_E_DEVICE, casim::last_error());
         *out = d;
     });
     if (rc != CASIM_OK && d) {
-        std::string keep = g_err;
+        std::string keep = casim::last_error();
         casim_dyad_destroy(d);
-        g_err = keep;
+        casim::last_error() = keep;
     }
     return rc;
 }

@@ -19,6 +19,25 @@ struct Err : std::runtime_error {
     Err(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
+// Message of the last failed ABI call on this thread (defined in casim_dyad.hip).
+std::string &last_error();
+
+// Run an ABI body, turning exceptions into status codes.
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        last_error().clear();
+        return CASIM_OK;
+    } catch (Err &e) {
+        last_error() = e.what();
+        return e.code;
+    } catch (std::exception &e) {
+        last_error() = e.what();
+        return CASIM_E_INVALID;
+    }
+}
+
 }  // namespace casim
 
 #define HIPCK(x)                                                                                  \

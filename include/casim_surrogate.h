@@ -1,0 +1,80 @@
+/*
+ * casim_surrogate.h -- learned mouth-[Ca2+] surrogate and the quasi-stationary
+ * release-site path it drives (part of libcasim_dyad.so).
+ *
+ * The quasi-stationary path is CaSim's event-driven release site
+ * (QuasyStationaryReleaseSite.cpp:89-133): between gating events the mouth
+ * [Ca2+] of every RyR is the quasi-steady value for the current open set;
+ * the reference obtains it by linear superposition of point-source solutions
+ * (StationaryDiffusionModel.cpp:3-11, LinearNaraghiNeher.cpp:9-75).  Here an
+ * ensemble of MLPs trained on the full reaction-diffusion solver predicts it,
+ * with the ensemble spread as an error estimate.
+ *
+ * The network: features -> width -> ... (depth hidden layers, SiLU) -> 1,
+ * BF16 weights and activations in the GEMMs, FP32 accumulation (MFMA), FP32
+ * master weights and Adam state.  All entry points return 0 or a CASIM_E*
+ * code (casim_dyad.h); casim_last_error() describes failures.
+ */
+#ifndef CASIM_SURROGATE_H
+#define CASIM_SURROGATE_H
+
+#include <stdint.h>
+
+#include "casim_dyad.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct CasimMLP CasimMLP;
+
+/* An ensemble of `members` MLPs with n_in inputs, `depth` hidden layers of
+ * `width` units and one output, initialised from `seed` (He-uniform
+ * weights, zero biases).  No reference counterpart. */
+int casim_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t members, uint64_t seed, CasimMLP **out);
+int casim_mlp_destroy(CasimMLP *m);
+
+/* Number of FP32 parameters of one member, and their layout: for each layer
+ * l = 0..depth, W_l row-major [in_l x out_l] followed by b_l [out_l]. */
+int64_t casim_mlp_param_count(const CasimMLP *m);
+int casim_mlp_get_params(CasimMLP *m, int32_t member, float *out);
+int casim_mlp_set_params(CasimMLP *m, int32_t member, const float *in);
+
+/* Feature and target normalisation applied inside train / predict:
+ * x' = (x - in_mean) / in_std, prediction = y' * out_std + out_mean. */
+int casim_mlp_set_normalization(CasimMLP *m, const float *in_mean, const float *in_std, float out_mean,
+                                float out_std);
+
+/* Training set, copied to the device: x [rows x n_in], y [rows] (raw units;
+ * normalised with the values above). */
+int casim_mlp_set_data(CasimMLP *m, const float *x, const float *y, int64_t rows);
+
+/* `epochs` passes of Adam (lr, beta1 0.9, beta2 0.999, eps 1e-8) over the
+ * training set in shuffled minibatches of `batch` rows, every member with its
+ * own shuffle.  loss_out[member] receives the mean squared error (normalised
+ * units) of the last epoch. */
+int casim_mlp_train(CasimMLP *m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float *loss_out);
+
+/* Gradient of the mean squared error (normalised units) of one member on
+ * rows of x/y (host arrays), written in the parameter layout above; also the
+ * loss.  For checking the device backward pass. */
+int casim_mlp_gradient(CasimMLP *m, int32_t member, const float *x, const float *y, int64_t rows, float *grad,
+                       float *loss);
+
+/* Ensemble prediction for rows of x (host arrays): mean and relative spread
+ * (standard deviation over members / mean) of exp(prediction), i.e. in
+ * concentration units when the target is ln [Ca].  Either output may be NULL;
+ * member_out [members x rows] receives every member's raw output. */
+int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, float *member_out);
+
+/* Plain C = op(A) op(B) in BF16 with FP32 accumulation, on device pointers
+ * (for tests of the MFMA kernel): A is M x K (K x M stored when trans_a),
+ * B is K x N (N x K stored when trans_b), leading dimensions in elements. */
+int casim_gemm_bf16(const void *a, int32_t lda, int32_t trans_a, const void *b, int32_t ldb, int32_t trans_b,
+                    float *c, int32_t ldc, int32_t M, int32_t N, int32_t K, float beta, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASIM_SURROGATE_H */

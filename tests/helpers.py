@@ -37,3 +37,26 @@ def rel_err(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.maximum(np.abs(b), 1e-300)
     return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+class DeviceBuffer:
+    """Raw device memory through the HIP runtime (for kernel-level tests)."""
+
+    def __init__(self, host):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so")
+        self.host = np.ascontiguousarray(host)
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.host.nbytes, 1))) == 0
+        assert self.hip.hipMemcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p), C.c_size_t(self.host.nbytes), 1) == 0
+
+    def read(self):
+        out = np.empty_like(self.host)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(self.C.c_void_p), self.ptr, self.C.c_size_t(out.nbytes), 2) == 0
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self.hip.hipFree(self.ptr)
+            self.ptr = None
