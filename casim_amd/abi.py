@@ -119,11 +119,14 @@ def _declare(lib):
 
 
 def header_symbols():
-    """Names of every function include/casim_dyad.h declares."""
+    """Names of every function the headers under include/ declare."""
+    import glob
     import re
-    hdr = os.path.join(os.path.dirname(HERE), "include", "casim_dyad.h")
-    text = open(hdr).read()
-    return sorted(set(re.findall(r"\b(casim_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for hdr in glob.glob(os.path.join(os.path.dirname(HERE), "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(hdr).read(), flags=re.S)
+        names |= set(re.findall(r"\b(casim_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def load():

@@ -28,6 +28,7 @@ FLAGS = [
 UNITS = [
     ("casim_dyad.hip", []),
     ("mlp.hip", []),
+    ("qs.hip", []),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=1", "-DCASIM_SWEEP_DIM=3"]),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=1", "-DCASIM_SWEEP_DIM=2"]),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=0", "-DCASIM_SWEEP_DIM=3"]),

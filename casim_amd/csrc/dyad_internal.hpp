@@ -105,3 +105,13 @@ void sweep_launch_f64_2d(CasimDyad *d, double dt);
 void sweep_launch_f32_3d(CasimDyad *d, double dt);
 void sweep_launch_f32_2d(CasimDyad *d, double dt);
 }  // namespace casim
+
+struct CasimMLP;
+namespace casim {
+// Device-side ensemble forward (mlp.hip): out [members x rows], normalised units.
+void mlp_forward_device(CasimMLP *m, const float *x, int64_t rows, float *out, hipStream_t stream);
+int mlp_members(const CasimMLP *m);
+int mlp_inputs(const CasimMLP *m);
+float mlp_out_mean(const CasimMLP *m);
+float mlp_out_std(const CasimMLP *m);
+}  // namespace casim

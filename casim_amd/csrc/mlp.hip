@@ -302,6 +302,46 @@ void CasimMLP::backward(int e, int B) {
     }
 }
 
+namespace casim {
+
+// Every member's output (normalised units) for `rows` raw feature rows on the
+// device: x [rows x n_in] -> out [members x rows].  Runs on `stream` after
+// the MLP's own stream has drained.
+void mlp_forward_device(CasimMLP *m, const float *x, int64_t rows, float *out, hipStream_t stream) {
+    m->use();
+    HIPCK(hipStreamSynchronize(m->stream));
+    hipStream_t keep = m->stream;
+    m->stream = stream;
+    try {
+        const int chunk = (int)std::min<int64_t>(rows, 65536);
+        m->reserve(chunk);
+        for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+            const int B = (int)std::min<int64_t>(chunk, rows - r0);
+            const size_t n = (size_t)B * m->n_in;
+            hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                               x + r0 * m->n_in, (const float *)nullptr, (const int *)nullptr, B, m->n_in, m->mean,
+                               m->istd, 0.f, 1.f, m->Xl[0], (float *)nullptr);
+            HIPCK(hipGetLastError());
+            for (int e = 0; e < m->members; ++e) {
+                m->forward(e, B);
+                HIPCK(hipMemcpyAsync(out + (size_t)e * rows + r0, m->Zl[m->depth], sizeof(float) * B,
+                                     hipMemcpyDeviceToDevice, stream));
+            }
+        }
+    } catch (...) {
+        m->stream = keep;
+        throw;
+    }
+    m->stream = keep;
+}
+
+int mlp_members(const CasimMLP *m) { return m->members; }
+int mlp_inputs(const CasimMLP *m) { return m->n_in; }
+float mlp_out_mean(const CasimMLP *m) { return m->ym; }
+float mlp_out_std(const CasimMLP *m) { return m->ys; }
+
+}  // namespace casim
+
 extern "C" {
 
 int casim_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t members, uint64_t seed, CasimMLP **out) {

@@ -1,0 +1,543 @@
+// qs.hip -- quasi-stationary release sites on the device, with the mouth
+// [Ca2+] from the surrogate ensemble (include/casim_surrogate.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/casim_surrogate.h"
+#include "dyad_internal.hpp"
+
+using namespace casim;
+
+namespace casim {
+
+constexpr int QS_NEAR = 16;
+constexpr int QS_F = 8 + 3 * QS_NEAR;  // features per RyR
+constexpr int QS_MAXCH = 128;
+
+enum QsStat { QS_AMP = 0, QS_TTP, QS_TTE, QS_Q, QS_TCUR, QS_NCUR, QS_EVENTS, QS_NSTAT };
+
+struct QsDev {
+    int S;
+    const int *off;
+    const float *x, *y;  // lattice units relative to the cleft centre
+    float half;          // half extent in lattice units
+    int *state;
+    uint32_t *rng;
+    double *t;
+    int *active;   // 1 while the site evolves
+    int *waiting;  // 0 no, 1 needs the full solver, 2 values supplied
+    int *crow;     // [S] first surrogate row of the site this round, -1 if none
+    int R;         // surrogate rows this round (member stride of out)
+    double *stats;
+    const double *jsr;
+    float *X;        // [channels x QS_F]
+    float *out;      // [members x channels]
+    float *ca;       // [channels] mouth [Ca2+] in use
+    float *spread;   // [channels]
+    float *ovr;      // [channels] supplied by the full solver
+    int model;
+    double par[NPAR];
+    double mg;
+    double t_max;
+    float thresh;
+    int members;
+    float ym, ys;
+};
+
+// Features of every channel of every active site: lanes over channels, the
+// 16 nearest open channels kept by insertion (stable for equal distances,
+// like a stable argsort).
+__global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
+    __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
+    __shared__ int so[QS_MAXCH];
+    const int s = blockIdx.x;
+    const int base = q.crow[s];
+    if (base < 0) return;
+    const int off = q.off[s], n = q.off[s + 1] - off;
+    for (int i = threadIdx.x; i < n; i += 64) {
+        sx[i] = q.x[off + i];
+        sy[i] = q.y[off + i];
+        so[i] = q.state[off + i] == 1;
+    }
+    __syncthreads();
+    int nopen = 0;
+    for (int j = 0; j < n; ++j) nopen += so[j];
+    for (int i = threadIdx.x; i < n; i += 64) {
+        float *f = q.X + (size_t)(base + i) * QS_F;
+        const float xi = sx[i], yi = sy[i];
+        float rk[QS_NEAR], dxk[QS_NEAR], dyk[QS_NEAR];
+#pragma unroll
+        for (int k = 0; k < QS_NEAR; ++k) {
+            rk[k] = INFINITY;
+            dxk[k] = 0.f;
+            dyk[k] = 0.f;
+        }
+        float sinv = 0.f, sexp = 0.f;
+        for (int j = 0; j < n; ++j) {
+            if (!so[j] || j == i) continue;
+            const float dx = sx[j] - xi, dy = sy[j] - yi;
+            const float r = sqrtf(dx * dx + dy * dy);
+            sinv += 1.f / r;
+            sexp += expf(-r);
+            if (r < rk[QS_NEAR - 1]) {
+                rk[QS_NEAR - 1] = r;
+                dxk[QS_NEAR - 1] = dx;
+                dyk[QS_NEAR - 1] = dy;
+#pragma unroll
+                for (int k = QS_NEAR - 1; k > 0; --k) {
+                    if (rk[k] < rk[k - 1]) {
+                        float t0 = rk[k]; rk[k] = rk[k - 1]; rk[k - 1] = t0;
+                        t0 = dxk[k]; dxk[k] = dxk[k - 1]; dxk[k - 1] = t0;
+                        t0 = dyk[k]; dyk[k] = dyk[k - 1]; dyk[k - 1] = t0;
+                    }
+                }
+            }
+        }
+        f[0] = (float)so[i];
+        f[1] = (float)(q.jsr[s] / 1000.0);
+        f[2] = nopen / 100.f;
+        f[3] = sinv;
+        f[4] = sexp;
+        f[5] = xi;
+        f[6] = yi;
+        f[7] = q.half - fmaxf(fabsf(xi), fabsf(yi));
+#pragma unroll
+        for (int k = 0; k < QS_NEAR; ++k) {
+            const bool ok = rk[k] < INFINITY;
+            f[8 + 3 * k] = ok ? dxk[k] : 0.f;
+            f[9 + 3 * k] = ok ? dyk[k] : 0.f;
+            f[10 + 3 * k] = ok ? 1.f / rk[k] : 0.f;
+        }
+    }
+}
+
+// Ensemble mean / spread of every channel of active sites, then one Gillespie
+// event per site unless the spread asks for the full solver.
+__global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
+    __shared__ double rate[QS_MAXCH];
+    __shared__ float wmax[64];
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int base = q.crow[s];
+    if (base < 0) return;
+    const int off = q.off[s], n = q.off[s + 1] - off;
+    const int wait = q.waiting[s];
+    float smax = 0.f;
+    for (int i = lane; i < n; i += 64) {
+        const size_t row = off + i;
+        float ca, sp;
+        if (wait == 2) {
+            ca = q.ovr[row];
+            sp = 0.f;
+        } else {
+            float sum = 0.f, sum2 = 0.f;
+            for (int e = 0; e < q.members; ++e) {
+                const float v = expf(q.out[(size_t)e * q.R + base + i] * q.ys + q.ym);
+                sum += v;
+                sum2 += v * v;
+            }
+            ca = sum / q.members;
+            const float var = fmaxf(0.f, sum2 / q.members - ca * ca);
+            sp = ca > 0.f ? sqrtf(var) / ca : 0.f;
+        }
+        q.ca[row] = ca;
+        q.spread[row] = sp;
+        smax = fmaxf(smax, sp);
+        const int st = q.state[row];
+        rate[i] = channel_rate(q.model, q.par, st, (double)ca, q.mg);
+    }
+    wmax[lane] = smax;
+    __syncthreads();
+    if (lane != 0) return;
+    float m = 0.f;
+    for (int k = 0; k < 64; ++k) m = fmaxf(m, wmax[k]);
+    if (wait == 0 && m > q.thresh) {
+        q.waiting[s] = 1;
+        return;
+    }
+    q.waiting[s] = 0;
+    // Gillespie step (QuasyStationaryReleaseSite.cpp:101-128)
+    uint32_t S6[6];
+    for (int k = 0; k < 6; ++k) S6[k] = q.rng[s * 6 + k];
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) total += rate[i];
+    double *st = q.stats + (size_t)s * QS_NSTAT;
+    if (!(total > 0.0)) {
+        q.active[s] = 0;
+        return;
+    }
+    const double u1 = mrg::draw(S6);
+    const double dtw = -log(u1) * (1.0 / total);
+    const double t = q.t[s] + dtw;
+    const double r = mrg::draw(S6);
+    double pc = 0.0;
+    int idx = 0;
+    for (int i = 0; i < n; ++i) {
+        const double w = rate[i] / total;
+        if (pc <= r && r <= pc + w) {
+            idx = i;
+            break;
+        }
+        pc += w;
+    }
+    (void)mrg::draw(S6);  // MakeTransition(runif())
+    const int was = q.state[off + idx];
+    q.state[off + idx] = was ^ 1;
+    const double ncur = st[QS_NCUR];
+    const double nnew = ncur + (was ? -1.0 : 1.0);
+    st[QS_Q] += ncur * (t - st[QS_TCUR]);
+    if (nnew > st[QS_AMP]) {
+        st[QS_AMP] = nnew;
+        st[QS_TTP] = t;
+    }
+    st[QS_TCUR] = t;
+    st[QS_TTE] = t;
+    st[QS_NCUR] = nnew;
+    st[QS_EVENTS] += 1.0;
+    q.t[s] = t;
+    for (int k = 0; k < 6; ++k) q.rng[s * 6 + k] = S6[k];
+    if (nnew < 1.0 || t >= q.t_max) q.active[s] = 0;
+}
+
+// InitOpening of every site (QuasyStationaryReleaseSite.cpp:91-97).
+__global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t *__restrict__ rng0,
+                          const uint32_t *__restrict__ jump) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= q.S) return;
+    const int off = q.off[s], n = q.off[s + 1] - off;
+    for (int k = 0; k < n; ++k) q.state[off + k] = (k == init[s]) ? 1 : 0;
+    uint32_t S6[6], o6[6];
+    for (int k = 0; k < 6; ++k) S6[k] = rng0[s * 6 + k];
+    const uint32_t *J = jump + (size_t)(n + 1) * 18;
+    mrg::matvec(J, S6, o6, mrg::C1, mrg::M1);
+    mrg::matvec(J + 9, S6 + 3, o6 + 3, mrg::C2, mrg::M2);
+    for (int k = 0; k < 6; ++k) q.rng[s * 6 + k] = o6[k];
+    q.t[s] = 0.0;
+    q.active[s] = q.t_max > 0.0;
+    q.waiting[s] = 0;
+    double *st = q.stats + (size_t)s * QS_NSTAT;
+    for (int k = 0; k < QS_NSTAT; ++k) st[k] = 0.0;
+    st[QS_AMP] = 1.0;
+    st[QS_NCUR] = 1.0;
+}
+
+// Rows of this round: every site with `all`, else the active sites that do
+// not wait for the full solver, packed in site order (one block).  cnt[0..2]:
+// active sites, waiting sites, rows.
+__global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restrict__ cnt) {
+    __shared__ int part[1024];
+    __shared__ int pa[1024], pw[1024];
+    const int t = threadIdx.x;
+    const int per = (q.S + 1023) / 1024;
+    const int s0 = t * per, s1 = min(q.S, s0 + per);
+    int rows = 0, na = 0, nw = 0;
+    for (int s = s0; s < s1; ++s) {
+        const bool act = q.active[s] != 0, w = act && q.waiting[s] == 1;
+        na += act;
+        nw += w;
+        if (all || (act && !w)) rows += q.off[s + 1] - q.off[s];
+    }
+    part[t] = rows;
+    pa[t] = na;
+    pw[t] = nw;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int r = part[t] - rows;
+    for (int s = s0; s < s1; ++s) {
+        const bool act = q.active[s] != 0, w = act && q.waiting[s] == 1;
+        if (all || (act && !w)) {
+            q.crow[s] = r;
+            r += q.off[s + 1] - q.off[s];
+        } else {
+            q.crow[s] = -1;
+        }
+    }
+    if (t == 0) {
+        int ta = 0, tw = 0;
+        for (int k = 0; k < 1024; ++k) {
+            ta += pa[k];
+            tw += pw[k];
+        }
+        cnt[0] = ta;
+        cnt[1] = tw;
+        cnt[2] = part[1023];
+    }
+}
+
+}  // namespace casim
+
+struct CasimQS {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    QsDev q{};
+    int nch = 0;
+    std::vector<int> h_off, h_init;
+    std::vector<uint32_t> h_rng0;
+    int *d_init = nullptr, *d_count = nullptr;
+    uint32_t *d_rng0 = nullptr, *d_jump = nullptr;
+    std::vector<void *> bufs;
+    void use() { HIPCK(hipSetDevice(dev)); }
+    template <class T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        HIPCK(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
+        bufs.push_back(p);
+        return (T *)p;
+    }
+};
+
+extern "C" {
+
+int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
+    CasimQS *h = nullptr;
+    int rc = guard([&] {
+        REQUIRE(c && out && c->n_sites >= 1 && c->site_channel_offset && c->channel_x && c->channel_y && c->jsr &&
+                    c->rng_state && c->init_channel && c->params,
+                CASIM_E_INVALID, "bad quasi-stationary configuration");
+        REQUIRE(c->model >= 1 && c->model <= 3, CASIM_E_INVALID, "unknown kinetic model");
+        REQUIRE(c->pitch > 0 && c->half_extent > 0, CASIM_E_INVALID, "pitch and extent must be > 0");
+        *out = nullptr;
+        h = new CasimQS();
+        HIPCK(hipGetDevice(&h->dev));
+        HIPCK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        const int S = c->n_sites;
+        h->h_off.assign(c->site_channel_offset, c->site_channel_offset + S + 1);
+        REQUIRE(h->h_off[0] == 0, CASIM_E_INVALID, "site_channel_offset[0] must be 0");
+        for (int s = 0; s < S; ++s) {
+            const int n = h->h_off[s + 1] - h->h_off[s];
+            REQUIRE(n >= 1 && n <= MAX_SITE_CHANNELS, CASIM_E_INVALID, "a site needs 1..127 channels");
+            REQUIRE(c->init_channel[s] >= 0 && c->init_channel[s] < n, CASIM_E_RANGE, "init channel out of range");
+        }
+        h->nch = h->h_off[S];
+        std::vector<float> x(h->nch), y(h->nch);
+        for (int k = 0; k < h->nch; ++k) {
+            x[k] = (float)((c->channel_x[k] - c->half_extent) / c->pitch);
+            y[k] = (float)((c->channel_y[k] - c->half_extent) / c->pitch);
+        }
+        h->h_rng0.resize((size_t)S * 6);
+        for (size_t i = 0; i < h->h_rng0.size(); ++i) {
+            REQUIRE(c->rng_state[i] < (i % 6 < 3 ? mrg::M1 : mrg::M2), CASIM_E_INVALID, "bad RNG state");
+            h->h_rng0[i] = (uint32_t)c->rng_state[i];
+        }
+        h->h_init.assign(c->init_channel, c->init_channel + S);
+        // jump table A^(2p), as the dyad handle builds it
+        std::vector<uint32_t> jump((size_t)JUMP_N * 18);
+        {
+            uint32_t A1[9], A2[9], S1[9], S2[9], P1[9], P2[9];
+            mrg::base_matrices(A1, A2);
+            mrg::matmul(A1, A1, S1, mrg::C1, mrg::M1);
+            mrg::matmul(A2, A2, S2, mrg::C2, mrg::M2);
+            mrg::identity(P1);
+            mrg::identity(P2);
+            for (int p = 0; p < JUMP_N; ++p) {
+                std::copy(P1, P1 + 9, &jump[(size_t)p * 18]);
+                std::copy(P2, P2 + 9, &jump[(size_t)p * 18 + 9]);
+                mrg::matmul(S1, P1, P1, mrg::C1, mrg::M1);
+                mrg::matmul(S2, P2, P2, mrg::C2, mrg::M2);
+            }
+        }
+        QsDev &q = h->q;
+        q.S = S;
+        int *off = h->alloc<int>(S + 1);
+        float *dx = h->alloc<float>(h->nch), *dy = h->alloc<float>(h->nch);
+        double *jsr = h->alloc<double>(S);
+        HIPCK(hipMemcpyAsync(off, h->h_off.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(dx, x.data(), sizeof(float) * h->nch, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(dy, y.data(), sizeof(float) * h->nch, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(jsr, c->jsr, sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
+        q.off = off;
+        q.x = dx;
+        q.y = dy;
+        q.jsr = jsr;
+        q.half = (float)(c->half_extent / c->pitch);
+        q.state = h->alloc<int>(h->nch);
+        q.rng = h->alloc<uint32_t>((size_t)S * 6);
+        q.t = h->alloc<double>(S);
+        q.active = h->alloc<int>(S);
+        q.waiting = h->alloc<int>(S);
+        q.stats = h->alloc<double>((size_t)S * QS_NSTAT);
+        q.X = h->alloc<float>((size_t)h->nch * QS_F);
+        q.ca = h->alloc<float>(h->nch);
+        q.spread = h->alloc<float>(h->nch);
+        q.ovr = h->alloc<float>(h->nch);
+        q.out = nullptr;
+        q.model = c->model;
+        for (int k = 0; k < NPAR; ++k) q.par[k] = c->params[k];
+        q.t_max = c->t_max;
+        q.mg = c->mg;
+        q.thresh = (float)c->spread_threshold;
+        h->d_init = h->alloc<int>(S);
+        h->d_count = h->alloc<int>(4);
+        q.crow = h->alloc<int>(S);
+        h->d_rng0 = h->alloc<uint32_t>((size_t)S * 6);
+        h->d_jump = h->alloc<uint32_t>(jump.size());
+        HIPCK(hipMemcpyAsync(h->d_init, h->h_init.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(h->d_rng0, h->h_rng0.data(), sizeof(uint32_t) * S * 6, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(h->d_jump, jump.data(), sizeof(uint32_t) * jump.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemsetAsync(q.ca, 0, sizeof(float) * h->nch, h->stream));
+        HIPCK(hipMemsetAsync(q.spread, 0, sizeof(float) * h->nch, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+        *out = h;
+        if (casim_qs_reset(h) != CASIM_OK) throw Err(CASIM_E_DEVICE, last_error());
+    });
+    if (rc != CASIM_OK && h) {
+        std::string keep = last_error();
+        casim_qs_destroy(h);
+        last_error() = keep;
+        if (out) *out = nullptr;
+    }
+    return rc;
+}
+
+int casim_qs_destroy(CasimQS *h) {
+    if (!h) return CASIM_OK;
+    (void)hipSetDevice(h->dev);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->bufs) (void)hipFree(p);
+    if (h->q.out) (void)hipFree(h->q.out);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CASIM_OK;
+}
+
+int casim_qs_reset(CasimQS *h) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        h->use();
+        hipLaunchKernelGGL(k_qs_init, dim3((h->q.S + 127) / 128), dim3(128), 0, h->stream, h->q, h->d_init, h->d_rng0,
+                           h->d_jump);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch, int32_t *rounds,
+                 int32_t *n_active, int32_t *n_waiting) {
+    return guard([&] {
+        REQUIRE(h && m, CASIM_E_INVALID, "null handle");
+        REQUIRE(mlp_inputs(m) == QS_F, CASIM_E_INVALID, "the surrogate must take 56 features");
+        h->use();
+        const int E = mlp_members(m);
+        if (!h->q.out || h->q.members != E) {
+            if (h->q.out) (void)hipFree(h->q.out);
+            h->q.out = nullptr;
+            HIPCK(hipMalloc(&h->q.out, sizeof(float) * (size_t)E * h->nch));
+        }
+        h->q.members = E;
+        h->q.ym = mlp_out_mean(m);
+        h->q.ys = mlp_out_std(m);
+        const int batch = std::max(1, (int)wait_batch);
+        int cnt[4] = {0, 0, 0, 0};
+        auto plan = [&] {
+            hipLaunchKernelGGL(k_qs_plan, dim3(1), dim3(1024), 0, h->stream, h->q, 0, h->d_count);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(cnt, h->d_count, sizeof(int) * 3, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(hipStreamSynchronize(h->stream));
+        };
+        plan();
+        int r = 0;
+        while (r < max_rounds && cnt[0] > 0 && cnt[1] < batch && cnt[1] < cnt[0]) {
+            h->q.R = cnt[2];
+            hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+            HIPCK(hipGetLastError());
+            mlp_forward_device(m, h->q.X, cnt[2], h->q.out, h->stream);
+            hipLaunchKernelGGL(k_qs_event, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+            HIPCK(hipGetLastError());
+            ++r;
+            plan();
+        }
+        if (rounds) *rounds = r;
+        if (n_active) *n_active = cnt[0];
+        if (n_waiting) *n_waiting = cnt[1];
+    });
+}
+
+int casim_qs_get_waiting(CasimQS *h, int32_t *flags, int32_t *states) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        h->use();
+        const int S = h->q.S;
+        std::vector<int> w(S), a(S), st(std::max(1, h->nch));
+        HIPCK(hipMemcpyAsync(w.data(), h->q.waiting, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipMemcpyAsync(a.data(), h->q.active, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipMemcpyAsync(st.data(), h->q.state, sizeof(int) * h->nch, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+        int M = 0;
+        for (int s = 0; s < S; ++s) M = std::max(M, h->h_off[s + 1] - h->h_off[s]);
+        if (flags)
+            for (int s = 0; s < S; ++s) flags[s] = (a[s] && w[s] == 1) ? 1 : 0;
+        if (states)
+            for (int s = 0; s < S; ++s) {
+                const int o = h->h_off[s], n = h->h_off[s + 1] - o;
+                for (int j = 0; j < M; ++j) states[(size_t)s * M + j] = j < n ? st[o + j] : -1;
+            }
+    });
+}
+
+int casim_qs_set_mouths(CasimQS *h, int32_t site, const float *ca) {
+    return guard([&] {
+        REQUIRE(h && ca, CASIM_E_INVALID, "null argument");
+        REQUIRE(site >= 0 && site < h->q.S, CASIM_E_RANGE, "site out of range");
+        h->use();
+        const int o = h->h_off[site], n = h->h_off[site + 1] - o;
+        const int two = 2;
+        HIPCK(hipMemcpyAsync(h->q.ovr + o, ca, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(h->q.waiting + site, &two, sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int casim_qs_get_stats(CasimQS *h, double *out) {
+    return guard([&] {
+        REQUIRE(h && out, CASIM_E_INVALID, "null argument");
+        h->use();
+        const int S = h->q.S;
+        std::vector<double> st((size_t)S * QS_NSTAT);
+        HIPCK(hipMemcpyAsync(st.data(), h->q.stats, sizeof(double) * st.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+        for (int s = 0; s < S; ++s) {
+            const double *a = &st[(size_t)s * QS_NSTAT];
+            double *o = out + (size_t)s * 6;
+            o[0] = a[QS_AMP];
+            o[1] = a[QS_TTP];
+            o[2] = a[QS_TTE];
+            o[3] = a[QS_Q];
+            o[4] = a[QS_TTE] > 0 ? a[QS_Q] / a[QS_TTE] : 0.0;
+            o[5] = a[QS_EVENTS];
+        }
+    });
+}
+
+int casim_qs_get_features(CasimQS *h, float *out) {
+    return guard([&] {
+        REQUIRE(h && out, CASIM_E_INVALID, "null argument");
+        h->use();
+        hipLaunchKernelGGL(k_qs_plan, dim3(1), dim3(1024), 0, h->stream, h->q, 1, h->d_count);
+        HIPCK(hipGetLastError());
+        hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, h->q.X, sizeof(float) * (size_t)h->nch * QS_F, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int casim_qs_get_mouths(CasimQS *h, float *mean, float *spread) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        h->use();
+        if (mean) HIPCK(hipMemcpyAsync(mean, h->q.ca, sizeof(float) * h->nch, hipMemcpyDeviceToHost, h->stream));
+        if (spread)
+            HIPCK(hipMemcpyAsync(spread, h->q.spread, sizeof(float) * h->nch, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

@@ -22,6 +22,16 @@ def _declare(lib):
         "casim_mlp_train": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
         "casim_mlp_gradient": (C.c_int, [P, C.c_int32, _fp, _fp, C.c_int64, _fp, _fp]),
         "casim_mlp_predict": (C.c_int, [P, _fp, C.c_int64, _fp, _fp, _fp]),
+        "casim_qs_create": (C.c_int, [P, C.POINTER(P)]),
+        "casim_qs_destroy": (C.c_int, [P]),
+        "casim_qs_reset": (C.c_int, [P]),
+        "casim_qs_run": (C.c_int, [P, P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
+        "casim_qs_get_waiting": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "casim_qs_set_mouths": (C.c_int, [P, C.c_int32, _fp]),
+        "casim_qs_get_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "casim_qs_get_features": (C.c_int, [P, _fp]),
+        "casim_qs_get_mouths": (C.c_int, [P, _fp, _fp]),
         "casim_gemm_bf16": (C.c_int, [P, C.c_int32, C.c_int32, P, C.c_int32, C.c_int32, P, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_float, P]),
     }
@@ -72,6 +82,7 @@ class MLP:
     def set_normalization(self, in_mean, in_std, out_mean, out_std):
         m = np.ascontiguousarray(in_mean, dtype=np.float32)
         s = np.ascontiguousarray(in_std, dtype=np.float32)
+        self.norm = (m.copy(), s.copy(), float(out_mean), float(out_std))
         abi.check(self.lib.casim_mlp_set_normalization(self.h, fptr(m), fptr(s), float(out_mean), float(out_std)))
 
     def fit_normalization(self, x, y):
@@ -107,3 +118,20 @@ class MLP:
         raw = np.empty((self.members, n), dtype=np.float32)
         abi.check(self.lib.casim_mlp_predict(self.h, fptr(x), n, fptr(mean), fptr(spread), fptr(raw)))
         return mean, spread, raw
+
+    def save(self, path):
+        """Shape, normalisation and every member's FP32 parameters (.npz)."""
+        m, s, ym, ys = getattr(self, "norm", (np.zeros(self.n_in), np.ones(self.n_in), 0.0, 1.0))
+        np.savez(path, shape=np.array([self.n_in, self.width, self.depth, self.members]),
+                 in_mean=m, in_std=s, out=np.array([ym, ys]),
+                 params=np.stack([self.params(e) for e in range(self.members)]))
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        n_in, width, depth, members = (int(v) for v in z["shape"])
+        net = cls(n_in, width, depth, members)
+        net.set_normalization(z["in_mean"], z["in_std"], float(z["out"][0]), float(z["out"][1]))
+        for e in range(members):
+            net.set_params(e, z["params"][e])
+        return net

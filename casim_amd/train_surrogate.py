@@ -1,0 +1,109 @@
+"""Train the mouth-[Ca2+] surrogate ensemble on the full solver and measure
+its error on held-out geometries.
+
+    python -m casim_amd.train_surrogate --batches 8 --sites 4096 --out casim_amd/surrogate_weights.npz
+
+Each batch is `sites` random dyads (RyR clusters of 8..100 on the 31 nm
+lattice, random open sets, jSR 200..1200 uM) relaxed for QS_STEPS clamped
+steps of the FP64 solver; the held-out batch uses geometry and open-set
+seeds no training batch uses.  Progress and results are JSON lines."""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import mlp, surrogate
+
+
+def dataset(DyadHIP, batches, sites, seed0=0, tuning=None, log=None):
+    xs, ys = [], []
+    for b in range(batches):
+        t0 = time.time()
+        x, y, _ = surrogate.generate(DyadHIP, n_sites=sites, seed=seed0 + b, tuning=tuning)
+        xs.append(x)
+        ys.append(y)
+        if log:
+            log({"stage": "data", "batch": b, "rows": int(len(y)), "s": round(time.time() - t0, 2)})
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def errors(net, x, y, threshold=0.02):
+    """Relative error of the ensemble mean in concentration units, overall
+    and on the rows whose spread stays under `threshold`."""
+    mean, spread, _ = net.predict(x)
+    true = np.exp(y.astype(np.float64))
+    rel = np.abs(mean - true) / true
+    ok = spread <= threshold
+    out = {"rows": int(len(y)), "rel_mean": float(rel.mean()), "rel_p50": float(np.median(rel)),
+           "rel_p95": float(np.quantile(rel, 0.95)), "rel_p99": float(np.quantile(rel, 0.99)),
+           "rel_max": float(rel.max()), "flagged_frac": float(1.0 - ok.mean())}
+    if ok.any():
+        out["rel_mean_unflagged"] = float(rel[ok].mean())
+        out["rel_p99_unflagged"] = float(np.quantile(rel[ok], 0.99))
+    return out
+
+
+def fit(x, y, width=256, depth=4, members=5, epochs=60, batch=1024, lr=1e-3, seed=1, log=None, schedule=(0.6, 0.85)):
+    net = mlp.MLP(x.shape[1], width=width, depth=depth, members=members, seed=seed)
+    net.fit_normalization(x, y)
+    net.set_data(x, y)
+    done = 0
+    # step decay: lr, lr/3 after schedule[0] of the epochs, lr/10 after schedule[1]
+    for frac_end, scale in ((schedule[0], 1.0), (schedule[1], 1.0 / 3.0), (1.0, 0.1)):
+        n = max(1, int(round(epochs * frac_end)) - done)
+        t0 = time.time()
+        loss = net.train(n, batch=batch, lr=lr * scale, seed=seed + 1000 * done)
+        done += n
+        if log:
+            log({"stage": "train", "epochs": done, "lr": lr * scale, "loss": [float(v) for v in loss],
+                 "s": round(time.time() - t0, 2)})
+    return net
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--batches", type=int, default=8)
+    ap.add_argument("--sites", type=int, default=4096)
+    ap.add_argument("--heldout-sites", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=256)
+    ap.add_argument("--depth", type=int, default=4)
+    ap.add_argument("--members", type=int, default=5)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--data", default=None, help="save / reuse the generated training set (.npz)")
+    a = ap.parse_args(argv)
+
+    def log(d):
+        print(json.dumps(d), flush=True)
+
+    from .dyad import DyadHIP
+    from . import autotune, workloads
+    spec = workloads.batched_dyads(n_sites=a.sites, clamped=True)
+    t = autotune.lookup(spec)
+    tuning = (t["planes"], t["threads"], t.get("prefetch", 1)) if t else None
+    if a.data:
+        try:
+            z = np.load(a.data, allow_pickle=False)
+            x, y = z["x"], z["y"]
+            log({"stage": "data", "loaded": a.data, "rows": int(len(y))})
+        except FileNotFoundError:
+            x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log)
+            np.savez(a.data, x=x, y=y)
+    else:
+        x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log)
+    xh, yh, _ = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, tuning=tuning)
+    net = fit(x, y, a.width, a.depth, a.members, a.epochs, a.batch, a.lr, log=log)
+    log({"stage": "heldout", **errors(net, xh, yh)})
+    log({"stage": "train_fit", **errors(net, x[:200_000], y[:200_000])})
+    if a.out:
+        net.save(a.out)
+        log({"stage": "saved", "path": a.out})
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

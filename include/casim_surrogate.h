@@ -67,6 +67,62 @@ int casim_mlp_gradient(CasimMLP *m, int32_t member, const float *x, const float 
  * member_out [members x rows] receives every member's raw output. */
 int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, float *member_out);
 
+/*
+ * Quasi-stationary release sites driven by the surrogate
+ * (QuasyStationaryReleaseSite::RunSimulation, QuasyStationaryReleaseSite.cpp:89-133):
+ * per site, InitOpening (one draw per channel, then one to open init_channel),
+ * then Gillespie events until no channel is open or t >= t_max, three draws
+ * per event: the waiting time -ln(u)/total rate (ParallelRng::rexp), the
+ * channel (cumulative rates, :113-122) and MakeTransition's draw.  The mouth
+ * [Ca2+] of every RyR for the current open set is the ensemble mean of the
+ * surrogate; a site whose ensemble spread exceeds spread_threshold at any RyR
+ * waits for the caller to supply the full solver's values
+ * (casim_qs_set_mouths) before its next event.
+ */
+typedef struct CasimQSConfig {
+    int32_t n_sites;
+    const int32_t *site_channel_offset; /* [n_sites + 1] */
+    const double *channel_x;            /* [channels] nm */
+    const double *channel_y;            /* [channels] nm */
+    double half_extent;                 /* nm: the cleft spans [0, 2 half_extent] in x and y */
+    double pitch;                       /* nm: length unit of the features (RyR lattice pitch) */
+    int32_t model;                      /* CASIM_MODEL_* */
+    const double *params;               /* [CASIM_MAX_MODEL_PARAMS] */
+    double mg;                          /* uM: the cleft's constant [Mg2+] (MWC / aHTG models) */
+    const double *jsr;                  /* [n_sites] uM */
+    const uint64_t *rng_state;          /* [n_sites * 6] */
+    const int32_t *init_channel;        /* [n_sites] */
+    double t_max;                       /* ms */
+    double spread_threshold;            /* relative ensemble spread that asks for the full solver */
+} CasimQSConfig;
+
+typedef struct CasimQS CasimQS;
+
+int casim_qs_create(const CasimQSConfig *cfg, CasimQS **out);
+int casim_qs_destroy(CasimQS *q);
+/* InitOpening of every site, clocks to 0, statistics cleared. */
+int casim_qs_reset(CasimQS *q);
+/* Up to max_rounds rounds of one event per active site that does not wait
+ * for the full solver.  Returns early when no site is active, when
+ * wait_batch sites wait, or when every active site waits.  rounds, n_active
+ * and n_waiting receive the rounds run and the counts after the last round
+ * (any may be NULL).  Calls on one handle must not overlap. */
+int casim_qs_run(CasimQS *q, CasimMLP *m, int32_t max_rounds, int32_t wait_batch, int32_t *rounds,
+                 int32_t *n_active, int32_t *n_waiting);
+/* Sites waiting for the full solver: flags [S] (1 = waiting) and the
+ * channel states of every site [S * M] (M = max channels, padding -1). */
+int casim_qs_get_waiting(CasimQS *q, int32_t *flags, int32_t *states);
+/* Mouth [Ca2+] of one waiting site's channels from the full solver; its next
+ * event uses them. */
+int casim_qs_set_mouths(CasimQS *q, int32_t site, const float *ca);
+/* Per site: amplitude, TTP, TTE, Q = integral of N dt, Q/TTE, events
+ * (IndChannelCREAnalyzer.cpp:89-137 columns from the event sequence). */
+int casim_qs_get_stats(CasimQS *q, double *out);
+/* Feature rows of every channel for the current open sets [channels x 56]. */
+int casim_qs_get_features(CasimQS *q, float *out);
+/* Last ensemble mean and relative spread of every channel. */
+int casim_qs_get_mouths(CasimQS *q, float *mean, float *spread);
+
 /* Plain C = op(A) op(B) in BF16 with FP32 accumulation, on device pointers
  * (for tests of the MFMA kernel): A is M x K (K x M stored when trans_a),
  * B is K x N (N x K stored when trans_b), leading dimensions in elements. */

@@ -1,0 +1,99 @@
+"""CPU restatement of the quasi-stationary release site's event loop
+(QuasyStationaryReleaseSite::RunSimulation, QuasyStationaryReleaseSite.cpp:89-133)
+and of the release-statistics columns IndChannelCREAnalyzer computes from an
+event record (IndChannelCREAnalyzer.cpp:69-117).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
+quasi-stationary path.  Pure Python loops over the oracle's MRG32k3a and
+channel rates (oracle.rng_draws, oracle.channel_rate); for small cases.
+
+The mouth [Ca2+] of an open set is a parameter (mouth_fn): the reference
+takes it from a StationaryDiffusionModel, the device path from the
+surrogate ensemble, so the tests pass the device's own values in.
+"""
+import math
+
+import numpy as np
+
+from . import oracle
+
+
+class Stream:
+    """One MRG32k3a stream drawn one U01d at a time (ParallelRng::runif)."""
+
+    def __init__(self, state):
+        self.state = np.ascontiguousarray(state, dtype=np.uint64).copy()
+
+    def runif(self):
+        u, self.state = oracle.rng_draws(self.state, 1)
+        return float(u[0])
+
+    def rexp(self, theta):
+        # ParallelRng.cpp:37-39
+        return -math.log(self.runif()) * theta
+
+
+def init_opening(n, init_channel, stream):
+    """QuasyStationaryReleaseSite.cpp:91-97: SetMacroState(0, runif) on every
+    channel, then SetMacroState(1, runif) on init_channel (two-state models
+    ignore the draw, the stream still advances)."""
+    states = np.zeros(n, dtype=np.int32)
+    for _ in range(n):
+        stream.runif()
+    stream.runif()
+    states[init_channel] = 1
+    return states
+
+
+def event(states, ca, model, params, mg, stream, t):
+    """One Gillespie event (QuasyStationaryReleaseSite.cpp:101-128) with mouth
+    [Ca2+] `ca` for the current states; returns (t, chosen channel).  The
+    channel flips (TwoStatePoModel / TwoStateModel::MakeTransition)."""
+    n = len(states)
+    rates = [oracle.channel_rate(model, params, int(states[i]), float(ca[i]), mg) for i in range(n)]
+    total = 0.0
+    for r in rates:
+        total += r
+    t = t + stream.rexp(1.0 / total)
+    u = stream.runif()
+    p = 0.0
+    idx = 0
+    for i in range(n):
+        w = rates[i] / total
+        if p <= u and u <= p + w:
+            idx = i
+            break
+        p += w
+    stream.runif()  # MakeTransition(runif())
+    states[idx] ^= 1
+    return t, idx
+
+
+def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn):
+    """Whole RunSimulation of one site: the record [(t, states)] and the
+    stream state after it."""
+    stream = Stream(rng_state)
+    states = init_opening(n, init_channel, stream)
+    t = 0.0
+    rec = [(t, states.copy())]
+    n_open = 1
+    while n_open >= 1 and t < t_max:
+        t, idx = event(states, mouth_fn(states), model, params, mg, stream, t)
+        n_open = int(states.sum())
+        rec.append((t, states.copy()))
+    return rec, stream.state
+
+
+def analyze(rec):
+    """amplitude, TTP, TTE, Q, Q/TTE and the number of events of one record
+    (IndChannelCREAnalyzer.cpp:69-117: amp / ttp from the first strict
+    maximum, Q = sum of N_j (t_{j+1} - t_j), TTE = time of the last record)."""
+    amp = ttp = q = 0.0
+    for j, (t, st) in enumerate(rec):
+        no = int(st.sum())
+        if no > amp:
+            amp, ttp = no, t
+        nxt = min(j + 1, len(rec) - 1)
+        q += no * (rec[nxt][0] - t)
+    tte = rec[-1][0]
+    return np.array([amp, ttp, tte, q, q / tte if tte > 0 else 0.0, len(rec) - 1])

@@ -1,0 +1,246 @@
+"""Quasi-stationary release sites (QuasyStationaryReleaseSite.cpp:89-133)
+driven by the surrogate: the oracle's event loop pinned by closed forms on
+the CPU, the device path checked against it event by event on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+from casim_amd import abi, workloads
+from oracle import oracle as O
+from oracle import qs_oracle as Q
+
+SHEEP = abi.CASIM_MODEL_SHEEP_CANNELL
+
+
+class FixedStream:
+    """Prescribed draws in place of MRG32k3a."""
+
+    def __init__(self, draws):
+        self.d = list(draws)
+
+    def runif(self):
+        return self.d.pop(0)
+
+    def rexp(self, theta):
+        return -math.log(self.runif()) * theta
+
+
+def sheep(kopen0, ncaopen, kclose0, ncaclose=0.0):
+    return [kopen0, ncaopen, 1e300, kclose0, ncaclose]
+
+
+def test_single_channel_closed_form():
+    # one open channel, constant kclose = 500 * 1e-3 / ms: InitOpening takes
+    # draws 0 and 1, the only event takes 2 (waiting time), 3 (choice), 4.
+    seed = O.stream_states([12345] * 6, 3)[2]
+    u, after = O.rng_draws(seed, 5)
+    rec, st = Q.run_site(1, 0, seed, SHEEP, sheep(0.0, 0.0, 500.0), 0.0, 100.0, lambda s: np.full(1, 7.0))
+    t = -math.log(u[2]) / 0.5
+    assert len(rec) == 2
+    assert rec[1][0] == pytest.approx(t, rel=1e-15)
+    assert list(rec[0][1]) == [1] and list(rec[1][1]) == [0]
+    assert np.array_equal(st, after)
+    a = Q.analyze(rec)
+    assert a[0] == 1 and a[1] == 0.0 and a[2] == pytest.approx(t) and a[3] == pytest.approx(t)
+    assert a[4] == pytest.approx(1.0) and a[5] == 1
+
+
+def test_event_choice_follows_cumulative_rates():
+    # ncaopen = 1: kopen proportional to [Ca]; Ca 1:2:3:4 gives weights
+    # 0.1, 0.2, 0.3, 0.4 (cumulative 0.1, 0.3, 0.6, 1.0) over closed channels.
+    par = sheep(1e6, 1.0, 500.0)
+    ca = np.array([1.0, 2.0, 3.0, 4.0])
+    for u, want in ((0.05, 0), (0.1, 0), (0.25, 1), (0.55, 2), (0.95, 3)):
+        st = np.zeros(4, dtype=np.int32)
+        t, idx = Q.event(st, ca, SHEEP, par, 0.0, FixedStream([0.5, u, 0.9]), 0.0)
+        assert idx == want, (u, idx)
+        assert st[want] == 1 and st.sum() == 1
+        total = 1e6 * 1e-3 * 1e-3 * ca.sum()
+        assert t == pytest.approx(math.log(2.0) / total, rel=1e-14)
+
+
+def test_event_flips_open_channels_too():
+    par = sheep(0.0, 0.0, 500.0)
+    st = np.array([1, 0, 1], dtype=np.int32)
+    # closed channel rate 0: weights 0.5, 0, 0.5
+    t, idx = Q.event(st, np.ones(3), SHEEP, par, 0.0, FixedStream([0.5, 0.7, 0.1]), 1.0)
+    assert idx == 2 and list(st) == [1, 0, 0]
+
+
+def test_analyze_hand_record():
+    rec = [(0.0, np.array([1, 0])), (1.0, np.array([1, 1])), (3.0, np.array([0, 1])), (4.0, np.array([0, 0]))]
+    a = Q.analyze(rec)
+    assert list(a) == [2.0, 1.0, 4.0, 6.0, 1.5, 3.0]
+
+
+def test_waiting_times_are_exponential():
+    # constant total rate 5 channels x 0.2 / ms: mean waiting time 1 ms
+    seed = O.stream_states([777] * 6, 1)[0]
+    s = Q.Stream(seed)
+    par = sheep(200.0, 0.0, 0.0)  # kopen0 * 1e-3 = 0.2 / ms
+    waits = []
+    for _ in range(400):
+        st = np.zeros(5, dtype=np.int32)
+        t, _ = Q.event(st, np.ones(5), SHEEP, par, 0.0, s, 0.0)
+        waits.append(t)
+    w = np.array(waits)
+    assert abs(w.mean() - 1.0) < 4 * 1.0 / math.sqrt(len(w))
+    # exponential: P(w > 1) = e^-1
+    assert abs((w > 1.0).mean() - math.exp(-1)) < 0.08
+
+
+# ---------------------------------------------------------------- GPU ----
+
+def _spec(n_sites, n_max=30, seed=5):
+    return workloads.batched_dyads(n_sites=n_sites, geometry_seed=seed, n_min=4, n_max=n_max)
+
+
+def _net(members=3, zero_bias=None, seed=3):
+    from casim_amd import mlp, surrogate
+    net = mlp.MLP(surrogate.N_FEAT, width=64, depth=2, members=members, seed=seed)
+    net.set_normalization(np.zeros(surrogate.N_FEAT), np.full(surrogate.N_FEAT, 4.0), math.log(15.0), 0.5)
+    if zero_bias is not None:
+        for e in range(members):
+            p = np.zeros(net.P, dtype=np.float32)
+            p[-1] = zero_bias
+            net.set_params(e, p)
+    return net
+
+
+def _par(spec):
+    ct = spec.channel_types[0]
+    from casim_amd import config
+    return [ct.params[n] for n in config.MODEL_PARAMS[ct.model]]
+
+
+def _site_rows(qs, s):
+    return slice(int(qs.off[s]), int(qs.off[s + 1]))
+
+
+@pytest.mark.gpu
+def test_device_features_match_numpy(gpu):
+    from casim_amd import surrogate
+    spec = _spec(48)
+    jsr = np.linspace(300.0, 1100.0, spec.n_sites)
+    net = _net()
+    qs = surrogate.QuasiStationarySites(spec, jsr, 5.0, net, spread_threshold=10.0)
+    for _ in range(6):
+        qs.Round(3)
+        st = qs.GetChannelsStates()
+        dev = qs.GetFeatures()
+        ref = surrogate.batch_features(spec, st, jsr)
+        assert dev.shape == ref.shape
+        np.testing.assert_allclose(dev, ref, rtol=2e-6, atol=2e-6)
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_constant_surrogate_whole_run_matches_oracle(gpu):
+    from casim_amd import surrogate
+    spec = _spec(24)
+    jsr = np.full(spec.n_sites, 800.0)
+    net = _net(zero_bias=0.6)
+    qs = surrogate.QuasiStationarySites(spec, jsr, 2.0, net, spread_threshold=0.02)
+    stats = qs.RunSimulation(fallback=False)
+    ca = qs.GetMouths()[0]
+    states = qs.GetChannelsStates()
+    par = _par(spec)
+    for s, site in enumerate(spec.sites):
+        n = len(site)
+        c = float(ca[_site_rows(qs, s)][0])
+        assert np.all(ca[_site_rows(qs, s)] == np.float32(c))
+        rec, _ = Q.run_site(n, spec.init_channel[s], spec.rng_states[s], SHEEP, par, 0.0, 2.0,
+                            lambda st: np.full(n, c))
+        want = Q.analyze(rec)
+        assert np.array_equal(states[s, :n], rec[-1][1]), s
+        np.testing.assert_array_equal(stats[s, [0, 5]], want[[0, 5]])
+        np.testing.assert_allclose(stats[s, 1:5], want[1:5], rtol=1e-12, atol=1e-15)
+    assert stats[:, 5].sum() > 5 * spec.n_sites  # the runs are not trivial
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_surrogate_events_match_oracle_event_by_event(gpu):
+    from casim_amd import surrogate
+    spec = _spec(16)
+    jsr = np.linspace(400.0, 1000.0, spec.n_sites)
+    net = _net(members=4)
+    qs = surrogate.QuasiStationarySites(spec, jsr, 3.0, net, spread_threshold=10.0)
+    par = _par(spec)
+    streams, states, t = [], [], np.zeros(spec.n_sites)
+    for s, site in enumerate(spec.sites):
+        streams.append(Q.Stream(spec.rng_states[s]))
+        states.append(Q.init_opening(len(site), spec.init_channel[s], streams[-1]))
+    active = np.ones(spec.n_sites, dtype=bool)
+    events = 0
+    for rnd in range(40):
+        feat = qs.GetFeatures()
+        mean, _, _ = net.predict(feat)
+        r, a, w = qs.Round(1)
+        if r == 0:
+            break
+        used, spread = qs.GetMouths()
+        dev_states = qs.GetChannelsStates()
+        stats = qs.GetStats()
+        for s in np.nonzero(active)[0]:
+            rows = _site_rows(qs, s)
+            np.testing.assert_allclose(used[rows], mean[rows], rtol=1e-5)
+            t[s], _ = Q.event(states[s], used[rows].astype(np.float64), SHEEP, par, 0.0, streams[s], t[s])
+            events += 1
+            n = len(spec.sites[s])
+            assert np.array_equal(dev_states[s, :n], states[s]), (rnd, s)
+            assert stats[s, 2] == pytest.approx(t[s], rel=1e-12)
+            if states[s].sum() == 0 or t[s] >= 3.0:
+                active[s] = False
+        assert a == active.sum()
+    assert events > 100
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_supplied_mouths_replace_the_surrogate(gpu):
+    from casim_amd import surrogate
+    spec = _spec(8)
+    jsr = np.full(spec.n_sites, 900.0)
+    net = _net(members=3)
+    qs = surrogate.QuasiStationarySites(spec, jsr, 5.0, net, spread_threshold=0.0)
+    par = _par(spec)
+    r, a, w = qs.Round(5)
+    assert (r, a, w) == (1, spec.n_sites, spec.n_sites)  # every site asks for the full solver
+    flags, _ = qs.Waiting()
+    assert flags.all()
+    assert qs.GetStats()[:, 5].sum() == 0  # no event was taken
+    streams = [Q.Stream(spec.rng_states[s]) for s in range(spec.n_sites)]
+    states = [Q.init_opening(len(site), spec.init_channel[s], streams[s]) for s, site in enumerate(spec.sites)]
+    for s, site in enumerate(spec.sites):
+        ca = np.linspace(5.0, 40.0, len(site)).astype(np.float32)
+        qs.SetMouths(s, ca)
+        Q.event(states[s], ca.astype(np.float64), SHEEP, par, 0.0, streams[s], 0.0)
+    r, a, w = qs.Round(1)
+    assert r == 1
+    dev = qs.GetChannelsStates()
+    for s, site in enumerate(spec.sites):
+        assert np.array_equal(dev[s, :len(site)], states[s])
+    assert qs.GetStats()[:, 5].sum() == spec.n_sites
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_fallback_uses_the_full_solver(gpu):
+    from casim_amd import surrogate
+    from casim_amd.dyad import DyadHIP
+    spec = _spec(4, n_max=12)
+    jsr = np.array([500.0, 700.0, 900.0, 1100.0])
+    net = _net(members=3)
+    qs = surrogate.QuasiStationarySites(spec, jsr, 5.0, net, spread_threshold=0.0)
+    qs.Round(1)
+    flags, states = qs.Waiting()
+    assert flags.all()
+    want = surrogate.qs_solve(DyadHIP, spec, states, jsr)
+    assert qs.resolve_waiting() == 4
+    qs.Round(1)
+    used, _ = qs.GetMouths()
+    for s, site in enumerate(spec.sites):
+        np.testing.assert_allclose(used[_site_rows(qs, s)], want[s, :len(site)].astype(np.float32), rtol=1e-6)
+    qs.close()
