@@ -122,6 +122,20 @@ void CasimDyad::launch_step(double dt) {
     cur ^= 1;
 }
 
+bool CasimDyad::pair_ok() const {
+    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || Q.on) return false;
+    return 4 * (size_t)P.n_grid * P.ny * P.nz * tsize <= 64 * 1024;
+}
+
+void CasimDyad::launch_pair(double dt) {
+    if (precision == CASIM_FP64)
+        P.dim == 3 ? sweep2_launch_f64_3d(this, dt) : sweep2_launch_f64_2d(this, dt);
+    else
+        P.dim == 3 ? sweep2_launch_f32_3d(this, dt) : sweep2_launch_f32_2d(this, dt);
+    HIPCK(hipGetLastError());
+    cur ^= 1;
+}
+
 void CasimDyad::launch_channels(int mode, double dt) {
     if (precision == CASIM_FP64)
         hipLaunchKernelGGL((k_channels<double>), dim3(P.S), dim3(64), 0, stream, (const double *)g[cur],
@@ -346,6 +360,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         // node-sorted list per site (stable: the last channel on a node wins)
         d->h_perm.resize(d->nch);
         d->h_snode.resize(d->nch);
+        d->h_sperm.resize(d->nch);
         for (int s = 0; s < P.S; ++s) {
             int o = d->h_off[s], n = d->h_off[s + 1] - o;
             std::vector<int> ord(n);
@@ -355,6 +370,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
             for (int r = 0; r < n; ++r) {
                 d->h_perm[o + ord[r]] = r;
                 d->h_snode[o + r] = d->h_node[o + ord[r]];
+                d->h_sperm[o + r] = ord[r];
             }
         }
         for (int t = 0; t < c->n_channel_types; ++t) {
@@ -404,6 +420,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         d->d_type = dalloc<int>(d->nch);
         d->d_perm = dalloc<int>(d->nch);
         d->d_snode = dalloc<int>(d->nch);
+        d->d_sperm = dalloc<int>(d->nch);
         d->d_state = dalloc<int>(d->nch);
         d->d_chans = dalloc<int>(P.S);
         d->d_rng = dalloc<uint32_t>((size_t)P.S * 6);
@@ -425,6 +442,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         up(d->d_type, types.data(), sizeof(int) * d->nch);
         up(d->d_perm, d->h_perm.data(), sizeof(int) * d->nch);
         up(d->d_snode, d->h_snode.data(), sizeof(int) * d->nch);
+        up(d->d_sperm, d->h_sperm.data(), sizeof(int) * d->nch);
         up(d->d_rng0, d->h_rng0.data(), sizeof(uint32_t) * P.S * 6);
         up(d->d_jump, jump.data(), sizeof(uint32_t) * jump.size());
         HIPCK(hipMemsetAsync(d->d_sJ, 0, sizeof(double) * P.ni * std::max(1, d->nch), d->stream));
@@ -436,6 +454,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         Ch.type = d->d_type;
         Ch.perm = d->d_perm;
         Ch.snode = d->d_snode;
+        Ch.sperm = d->d_sperm;
         Ch.state = d->d_state;
         Ch.rng = d->d_rng;
         Ch.jump = d->d_jump;
@@ -462,7 +481,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     d->clear_graphs();
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
-                    d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
+                    d->d_sperm, d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
                     d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -737,7 +756,8 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
             HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
             try {
                 const bool pool_rim = d->Q.on && d->Q.cyto;
-                for (int k = 0; k < k_steps; ++k) {
+                const bool pairs = d->pair_ok();
+                for (int k = 0; k < k_steps;) {
                     if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux
                         d->launch_rim(d->Q.cions, d->Q.cbb);
                         d->launch_efflux();
@@ -747,7 +767,16 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                     d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
                                            CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                        dt);
-                    d->launch_step(dt);
+                    if (pairs && k + 1 < k_steps) {
+                        // steps k and k+1 in one pass, then step k+1's channel
+                        // update from the mouth values the pass stored
+                        d->launch_pair(dt);
+                        d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE, dt);
+                        k += 2;
+                    } else {
+                        d->launch_step(dt);
+                        k += 1;
+                    }
                 }
                 if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
                 d->launch_efflux();
@@ -763,11 +792,12 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
             hipGraphExec_t exec;
             HIPCK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             HIPCK(hipGraphDestroy(graph));
+            const int flips = d->cur ^ cur0;
             d->cur = cur0;
-            it = d->graphs.emplace(key, exec).first;
+            it = d->graphs.emplace(key, std::make_pair(exec, flips)).first;
         }
-        HIPCK(hipGraphLaunch(it->second, d->stream));
-        d->cur = cur0 ^ (k_steps & 1);
+        HIPCK(hipGraphLaunch(it->second.first, d->stream));
+        d->cur = cur0 ^ it->second.second;
     });
 }
 
@@ -781,7 +811,11 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
         for (auto &e : ev) HIPCK(hipEventCreate(&e));
         try {
             const bool pool_rim = d->Q.on && d->Q.cyto;
-            for (int k = 0; k < k_steps; ++k) {
+            const bool pairs = d->pair_ok();
+            // per step: channel-kernel time in [4k, 4k+1], sweep time in [4k+2, 4k+3]
+            // (a two-step pass is timed on its first step, its second channel call
+            // on the second step)
+            for (int k = 0; k < k_steps;) {
                 if (pool_rim) {
                     d->launch_rim(d->Q.cions, d->Q.cbb);
                     d->launch_efflux();
@@ -792,8 +826,20 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                                    dt);
                 HIPCK(hipEventRecord(ev[4 * k + 1], d->stream));
                 HIPCK(hipEventRecord(ev[4 * k + 2], d->stream));
-                d->launch_step(dt);
-                HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
+                if (pairs && k + 1 < k_steps) {
+                    d->launch_pair(dt);
+                    HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
+                    HIPCK(hipEventRecord(ev[4 * (k + 1)], d->stream));
+                    d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE, dt);
+                    HIPCK(hipEventRecord(ev[4 * (k + 1) + 1], d->stream));
+                    HIPCK(hipEventRecord(ev[4 * (k + 1) + 2], d->stream));
+                    HIPCK(hipEventRecord(ev[4 * (k + 1) + 3], d->stream));
+                    k += 2;
+                } else {
+                    d->launch_step(dt);
+                    HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
+                    k += 1;
+                }
             }
             if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
             d->launch_efflux();
@@ -1073,6 +1119,17 @@ int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t thread
         d->clear_graphs();
     });
 }
+
+int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps) {
+    return guard([&] {
+        REQUIRE(d, CASIM_E_INVALID, "null handle");
+        REQUIRE(steps == 1 || steps == 2, CASIM_E_INVALID, "steps per pass must be 1 or 2");
+        d->steps_per_pass = steps;
+        d->clear_graphs();
+    });
+}
+
+int32_t casim_dyad_steps_per_pass(const CasimDyad *d) { return d ? (d->pair_ok() ? 2 : 1) : -1; }
 
 void *casim_dyad_stream(CasimDyad *d) { return d ? (void *)d->stream : nullptr; }
 

@@ -67,9 +67,9 @@ struct CasimDyad {
     int nch = 0, M = 0;
     double zext = 0, dx = 0, dy = 0, dz = 0, radius = 0, voltage = 0;
     double elem_volume = 0, grad_mult = 0, sxm = 0, sym = 0;
-    std::vector<int> h_off, h_node, h_perm, h_snode;
+    std::vector<int> h_off, h_node, h_perm, h_snode, h_sperm;
     std::vector<uint32_t> h_rng0;
-    int *d_off = nullptr, *d_node = nullptr, *d_type = nullptr, *d_perm = nullptr, *d_snode = nullptr,
+    int *d_off = nullptr, *d_node = nullptr, *d_type = nullptr, *d_perm = nullptr, *d_snode = nullptr, *d_sperm = nullptr,
         *d_state = nullptr, *d_chans = nullptr;
     uint32_t *d_rng = nullptr, *d_rng0 = nullptr, *d_jump = nullptr;
     double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
@@ -81,7 +81,9 @@ struct CasimDyad {
     int sweep_chunk = 0;      // interior x-planes per k_sweep workgroup
     int sweep_prefetch = 1;   // planes prefetched beyond x+1
     bool use_sweep = true;
-    std::map<std::tuple<int, double, int>, hipGraphExec_t> graphs;
+    int steps_per_pass = 2;   // 2: pairs of steps in one k_sweep2 pass where pair_ok()
+    // captured K-step graphs and the buffer flips each makes
+    std::map<std::tuple<int, double, int>, std::pair<hipGraphExec_t, int>> graphs;
 
     size_t grid_elems() const { return (size_t)P.n_grid * P.S * P.NS; }
     size_t face_elems() const { return (size_t)P.n_face * P.S * 2 * P.nFP; }
@@ -89,11 +91,13 @@ struct CasimDyad {
     void use() { HIPCK(hipSetDevice(dev)); }
 
     void launch_step(double dt);
+    bool pair_ok() const;
+    void launch_pair(double dt);
     void launch_channels(int mode, double dt);
     void launch_efflux();
     void launch_rim(const double *ions, const double *bound);
     void clear_graphs() {
-        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second.first);
         graphs.clear();
     }
 };
@@ -104,6 +108,10 @@ void sweep_launch_f64_3d(CasimDyad *d, double dt);
 void sweep_launch_f64_2d(CasimDyad *d, double dt);
 void sweep_launch_f32_3d(CasimDyad *d, double dt);
 void sweep_launch_f32_2d(CasimDyad *d, double dt);
+void sweep2_launch_f64_3d(CasimDyad *d, double dt);
+void sweep2_launch_f64_2d(CasimDyad *d, double dt);
+void sweep2_launch_f32_3d(CasimDyad *d, double dt);
+void sweep2_launch_f32_2d(CasimDyad *d, double dt);
 }  // namespace casim
 
 struct CasimMLP;

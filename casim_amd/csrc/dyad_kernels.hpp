@@ -48,6 +48,7 @@ struct Channels {
     const int *type;    // [nch]
     const int *perm;    // [nch] position of the channel in its site's node-sorted list
     const int *snode;   // [nch] node-sorted list of mouth nodes per site
+    const int *sperm;   // [nch] channel (index within its site) at each node-sorted position
     int *state;         // [nch]
     uint32_t *rng;      // [S*6]
     const uint32_t *jump;  // [JUMP_N*18]
@@ -72,6 +73,7 @@ enum ChannelMode {
     CH_ACCUMULATE = 16,
     CH_POOL_UPDATE = 32,   // SRPool::Update from the fresh site totals
     CH_POOL_ADVANCE = 64,  // CytosolPool::RunRD and SRPool::Run for the next step
+    CH_FROM_STORED = 128,  // take the mouth values from mouths[] (written by k_sweep2) instead of the grid
 };
 
 constexpr int MAXCU = 12;  // cytosol buffers (dyad buffers first)
@@ -358,6 +360,193 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Species layouts the sweeps can take at compile time: bound field b binds
+// ion bion[b], belongs to buffer bbuf[b], lives on the z-faces only when
+// bface[b] and sits in grid (or face) slot bslot[b]; ufaces[u]: buffer u
+// exists on the two z-faces only.  LY = 0 reads them from Params at run time.
+//  1: one ion; buffers A, B mobile, F on the z-faces (Ca-A, Ca-B, Ca-F)
+//  2: one ion; buffers A, B mobile, F fixed on every node
+template <int LY>
+struct Layout {
+    static constexpr bool fixed = false;
+    static constexpr int nu = MAXU;
+    static constexpr int bion[1] = {0}, bbuf[1] = {0}, bface[1] = {0}, bslot[1] = {0}, ufaces[1] = {0};
+};
+template <>
+struct Layout<1> {
+    static constexpr bool fixed = true;
+    static constexpr int nu = 3;
+    static constexpr int bion[3] = {0, 0, 0}, bbuf[3] = {0, 1, 2}, bface[3] = {0, 0, 1}, bslot[3] = {1, 2, 0};
+    static constexpr int ufaces[3] = {0, 0, 1};
+};
+template <>
+struct Layout<2> {
+    static constexpr bool fixed = true;
+    static constexpr int nu = 3;
+    static constexpr int bion[3] = {0, 0, 0}, bbuf[3] = {0, 1, 2}, bface[3] = {0, 0, 0}, bslot[3] = {1, 2, 3};
+    static constexpr int ufaces[3] = {0, 0, 0};
+};
+template <int LY> __device__ __forceinline__ int ly_bion(const Params &p, int b) {
+    if constexpr (Layout<LY>::fixed) return Layout<LY>::bion[b]; else return p.bion[b];
+}
+template <int LY> __device__ __forceinline__ int ly_bbuf(const Params &p, int b) {
+    if constexpr (Layout<LY>::fixed) return Layout<LY>::bbuf[b]; else return p.bbuf[b];
+}
+template <int LY> __device__ __forceinline__ int ly_bface(const Params &p, int b) {
+    if constexpr (Layout<LY>::fixed) return Layout<LY>::bface[b]; else return p.bface[b];
+}
+template <int LY> __device__ __forceinline__ int ly_bslot(const Params &p, int b) {
+    if constexpr (Layout<LY>::fixed) return Layout<LY>::bslot[b]; else return p.bslot[b];
+}
+template <int LY> __device__ __forceinline__ double ly_total(const Params &p, int u, int z) {
+    int faces;
+    if constexpr (Layout<LY>::fixed) faces = Layout<LY>::ufaces[u]; else faces = p.ufaces[u];
+    return (faces && p.dim == 3 && z != 0 && z != p.nz - 1) ? 0.0 : p.total[u];
+}
+
+// Updated values of one interior node in DyadRDCPU.cpp:404-492 order: the
+// Laplacian of every diffusing field (x-neighbours from registers, y and z
+// from the LDS plane L), the reactions with the free buffers seen by each
+// bound field, the channel current, the sarcolemmal factor at z = nz-1 (3D)
+// or everywhere (2D), the Euler step.  C0/Cm/Cp: the node and its x-1 / x+1
+// neighbours, every grid field; Fv: face values of the node.  Writes the new
+// grid values to G (every grid field) and, at face nodes, the new face values
+// to Fo.
+template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int LI_ = NI ? NI : MAXI>
+__device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], const T (&C0)[NG],
+                                            const T (&Cp)[NG], const T *L, int c, int P, int zl, int zr, int z,
+                                            bool face, const T (&Fv)[NF], const T (&J)[LI_], T dt, T ihx2, T ihy2, T ihz2,
+                                            T (&G)[NG], T (&Fo)[NF]) {
+    constexpr int LI = NI ? NI : MAXI;
+    constexpr int LB = NB ? NB : MAXB;
+    T Cv[LI], eC[LI], Bv[LB], eB[LB];
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        const T vc = C0[j];
+        Cv[j] = vc;
+        const T *u = L + j * P + c;
+        T lap = (Cp[j] - 2 * vc + Cm[j]) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
+        if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+        eC[j] = (T)p.Dion[j] * lap;
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        const int g = ly_bslot<LY>(p, b);
+        if (ly_bface<LY>(p, b)) {
+            T fv = (T)0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f == g) fv = Fv[f];
+            Bv[b] = face ? fv : (T)0;
+            eB[b] = 0;
+        } else {
+            T vc = (T)0, cxr = (T)0, cxl = (T)0;
+#pragma unroll
+            for (int gg = 0; gg < NG; ++gg)
+                if (gg == g) {
+                    vc = C0[gg];
+                    cxr = Cp[gg];
+                    cxl = Cm[gg];
+                }
+            Bv[b] = vc;
+            if (p.Db[b] != 0.0) {
+                const T *u = L + g * P + c;
+                T lap = (cxr - 2 * vc + cxl) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
+                if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+                eB[b] = (T)p.Db[b] * lap;
+            } else {
+                eB[b] = 0;
+            }
+        }
+    }
+    // free buffer seen by each bound field: total minus its buffer's bound
+    // fields, subtracted in bound order
+    T frb[LB];
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        T v = (T)ly_total<LY>(p, ly_bbuf<LY>(p, b), z);
+#pragma unroll
+        for (int b2 = 0; b2 < LB; ++b2) {
+            if (!NB && b2 >= p.nb) break;
+            if (ly_bbuf<LY>(p, b2) == ly_bbuf<LY>(p, b)) v -= Bv[b2];
+        }
+        frb[b] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        T Rr = 0;
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+            if (!NB && b >= p.nb) break;
+            if (ly_bion<LY>(p, b) != j) continue;
+            if (ly_bface<LY>(p, b) && !face) continue;
+            T R1 = (T)p.koff[b] * Bv[b] - (T)p.kon[b] * Cv[j] * frb[b];
+            eB[b] += -R1;
+            Rr += R1;
+        }
+        eC[j] += Rr;
+    }
+#pragma unroll
+    for (int j = 0; j < LI; ++j) {
+        if (!NI && j >= p.ni) break;
+        T e = eC[j] + J[j];
+        if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
+            T s1 = (T)p.K1[j] + Cv[j];
+            T s2 = (T)p.K2[j] + Cv[j];
+            e *= (T)1 / ((T)1 + (T)p.N1[j] * (T)p.K1[j] / (s1 * s1) + (T)p.N2[j] * (T)p.K2[j] / (s2 * s2));
+        }
+        G[j] = Cv[j] + e * dt;
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        const int g = ly_bslot<LY>(p, b);
+        const T v = Bv[b] + eB[b] * dt;
+        if (ly_bface<LY>(p, b)) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f == g) Fo[f] = v;
+        } else {
+#pragma unroll
+            for (int gg = 0; gg < NG; ++gg)
+                if (gg == g) G[gg] = v;
+        }
+    }
+}
+
+// Per-thread slot geometry of a plane (shared by both sweeps).
+template <int DIM, int SL>
+struct PlaneSlots {
+    int q[SL], zl[SL], zr[SL], fz[SL], yy[SL];
+    bool act[SL], inner[SL];
+    __device__ __forceinline__ void init(const Params &p, int P) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+            q[k] = threadIdx.x + k * blockDim.x;
+            act[k] = q[k] < P;
+            const int z = DIM == 3 ? q[k] % p.nz : 0;
+            yy[k] = DIM == 3 ? q[k] / p.nz : q[k];
+            inner[k] = act[k] && yy[k] >= 1 && yy[k] <= p.ny - 2;
+            zl[k] = DIM == 3 ? (z == 0 ? 1 : -1) : 0;
+            zr[k] = DIM == 3 ? (z == p.nz - 1 ? -1 : 1) : 0;
+            fz[k] = (DIM == 3 && act[k]) ? (z == 0 ? 0 : (z == p.nz - 1 ? 1 : -1)) : -1;
+        }
+    }
+};
+
+// First entry of a site's node-sorted channel list at or after plane x.
+__device__ __forceinline__ int first_channel_at(const int *__restrict__ snode, int lo, int hi, int key) {
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (snode[m] < key) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // The state sweep, plane-marching form.  One workgroup owns one x-chunk of one
 // site and walks its planes x0..x1-1 in order.  A plane (fixed x) is the
@@ -370,7 +559,7 @@ __device__ __forceinline__ void lds_barrier() {
 // two halo planes.  Units are laid out so that consecutive chunks of a site
 // land on the same XCD (blocks are dealt to XCDs round-robin).
 // Arithmetic and operation order as k_step (DyadRDCPU.cpp:404-506).
-template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int SL, int PD>
+template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL, int PD>
 __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__restrict__ gout,
                                                const T *__restrict__ fin, T *__restrict__ fout,
                                                const int *__restrict__ ch_off, const int *__restrict__ snode,
@@ -399,19 +588,8 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
     const T dt = (T)dt_;
     const T ihx2 = (T)p.ihx2, ihy2 = (T)p.ihy2, ihz2 = (T)p.ihz2;
 
-    int q[SL], zl[SL], zr[SL], fz[SL], yy[SL];
-    bool act[SL], inner[SL];
-#pragma unroll
-    for (int k = 0; k < SL; ++k) {
-        q[k] = threadIdx.x + k * blockDim.x;
-        act[k] = q[k] < P;
-        const int z = DIM == 3 ? q[k] % p.nz : 0;
-        yy[k] = DIM == 3 ? q[k] / p.nz : q[k];
-        inner[k] = act[k] && yy[k] >= 1 && yy[k] <= p.ny - 2;
-        zl[k] = DIM == 3 ? (z == 0 ? 1 : -1) : 0;
-        zr[k] = DIM == 3 ? (z == p.nz - 1 ? -1 : 1) : 0;
-        fz[k] = (DIM == 3 && act[k]) ? (z == 0 ? 0 : (z == p.nz - 1 ? 1 : -1)) : -1;
-    }
+    PlaneSlots<DIM, SL> ps;
+    ps.init(p, P);
 
     // R[r] holds plane x-1+r of every grid field; F[r] the face values of plane x+r
     T R[RS][SL][NG];
@@ -421,15 +599,16 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && act[k]) D[k][g] = gs[g * fs + (size_t)x * P + q[k]];
+                if (g < ng && ps.act[k]) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
     };
     auto load_face = [&](T (&D)[SL][NF], int x) {
 #pragma unroll
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int f = 0; f < NF; ++f)
-                if (f < nfc && fz[k] >= 0)
-                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)fz[k] * p.nFP + (size_t)x * p.ny + yy[k]];
+                if (f < nfc && ps.fz[k] >= 0)
+                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
+                                  ps.yy[k]];
     };
 #pragma unroll
     for (int r = 0; r < RS - 1; ++r)
@@ -438,17 +617,8 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
     for (int r = 0; r < PD; ++r)
         if (x0 + r < x1) load_face(F[r], x0 + r);
 
-    // first mouth of this chunk in the node-sorted list
-    int cur = ch_off[s];
+    int cur = first_channel_at(snode, ch_off[s], ch_off[s + 1], x0 * P);
     const int chi = ch_off[s + 1];
-    {
-        int e = chi;
-        const int key = x0 * P;
-        while (cur < e) {
-            const int m = (cur + e) >> 1;
-            if (snode[m] < key) cur = m + 1; else e = m;
-        }
-    }
 
     for (int x = x0; x < x1; ++x) {
         if (x + 1 + PD <= x1) load_plane(R[RS - 1], x + 1 + PD);
@@ -458,7 +628,7 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && act[k]) L[g * P + q[k]] = R[1][k][g];
+                if (g < ng && ps.act[k]) L[g * P + ps.q[k]] = R[1][k][g];
         lds_barrier();
 
         // channel currents on this plane (the last channel on a node wins)
@@ -474,7 +644,7 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
             const int qq = nd - x * P;
 #pragma unroll
             for (int k = 0; k < SL; ++k)
-                if (q[k] == qq)
+                if (ps.q[k] == qq)
 #pragma unroll
                     for (int j = 0; j < LI; ++j)
                         if (NI || j < p.ni) J[k][j] = (T)sJ[(size_t)j * nch + cur];
@@ -483,104 +653,22 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 
 #pragma unroll
         for (int k = 0; k < SL; ++k) {
-            if (!inner[k]) continue;
-            const int c = q[k];
-            const bool face = fz[k] >= 0;
+            if (!ps.inner[k]) continue;
+            const int c = ps.q[k];
             const int z = DIM == 3 ? c % p.nz : 0;
-            T Cv[LI], eC[LI], Bv[LB], eB[LB];
-#pragma unroll
-            for (int j = 0; j < LI; ++j) {
-                if (!NI && j >= p.ni) break;
-                const T vc = R[1][k][j];
-                Cv[j] = vc;
-                const T *u = L + j * P + c;
-                T lap = (R[2][k][j] - 2 * vc + R[0][k][j]) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
-                if (DIM == 3) lap = lap + (u[zr[k]] - 2 * vc + u[zl[k]]) * ihz2;
-                eC[j] = (T)p.Dion[j] * lap;
-            }
-#pragma unroll
-            for (int b = 0; b < LB; ++b) {
-                if (!NB && b >= p.nb) break;
-                const int g = p.bslot[b];
-                if (p.bface[b]) {
-                    T fv = (T)0;
-#pragma unroll
-                    for (int f = 0; f < NF; ++f)
-                        if (f == g) fv = F[0][k][f];
-                    Bv[b] = face ? fv : (T)0;
-                    eB[b] = 0;
-                } else {
-                    T vc = (T)0, cxr = (T)0, cxl = (T)0;
-#pragma unroll
-                    for (int gg = 0; gg < NG; ++gg)
-                        if (gg == g) {
-                            vc = R[1][k][gg];
-                            cxr = R[2][k][gg];
-                            cxl = R[0][k][gg];
-                        }
-                    Bv[b] = vc;
-                    if (p.Db[b] != 0.0) {
-                        const T *u = L + g * P + c;
-                        T lap = (cxr - 2 * vc + cxl) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
-                        if (DIM == 3) lap = lap + (u[zr[k]] - 2 * vc + u[zl[k]]) * ihz2;
-                        eB[b] = (T)p.Db[b] * lap;
-                    } else {
-                        eB[b] = 0;
-                    }
-                }
-            }
-            // free buffer seen by each bound field: total minus its buffer's
-            // bound fields, subtracted in bound order
-            T frb[LB];
-#pragma unroll
-            for (int b = 0; b < LB; ++b) {
-                if (!NB && b >= p.nb) break;
-                T v = (T)total_at(p, p.bbuf[b], z);
-#pragma unroll
-                for (int b2 = 0; b2 < LB; ++b2) {
-                    if (!NB && b2 >= p.nb) break;
-                    if (p.bbuf[b2] == p.bbuf[b]) v -= Bv[b2];
-                }
-                frb[b] = v;
-            }
-#pragma unroll
-            for (int j = 0; j < LI; ++j) {
-                if (!NI && j >= p.ni) break;
-                T Rr = 0;
-#pragma unroll
-                for (int b = 0; b < LB; ++b) {
-                    if (!NB && b >= p.nb) break;
-                    if (p.bion[b] != j) continue;
-                    if (p.bface[b] && !face) continue;
-                    T R1 = (T)p.koff[b] * Bv[b] - (T)p.kon[b] * Cv[j] * frb[b];
-                    eB[b] += -R1;
-                    Rr += R1;
-                }
-                eC[j] += Rr;
-            }
+            T G[NG], Fo[NF];
+            node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], L, c, P, ps.zl[k], ps.zr[k], z,
+                                                ps.fz[k] >= 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
             const size_t o = (size_t)x * P + c;
 #pragma unroll
-            for (int j = 0; j < LI; ++j) {
-                if (!NI && j >= p.ni) break;
-                T e = eC[j] + J[k][j];
-                if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
-                    T s1 = (T)p.K1[j] + Cv[j];
-                    T s2 = (T)p.K2[j] + Cv[j];
-                    e *= (T)1 / ((T)1 + (T)p.N1[j] * (T)p.K1[j] / (s1 * s1) + (T)p.N2[j] * (T)p.K2[j] / (s2 * s2));
-                }
-                os[j * fs + o] = Cv[j] + e * dt;
-            }
+            for (int g = 0; g < NG; ++g)
+                if (g < ng) os[g * fs + o] = G[g];
+            if (ps.fz[k] >= 0)
 #pragma unroll
-            for (int b = 0; b < LB; ++b) {
-                if (!NB && b >= p.nb) break;
-                if (p.bface[b]) {
-                    if (face)
-                        fout[((size_t)p.bslot[b] * p.S + s) * 2 * p.nFP + (size_t)fz[k] * p.nFP + (size_t)x * p.ny +
-                             yy[k]] = Bv[b] + eB[b] * dt;
-                } else {
-                    os[p.bslot[b] * fs + o] = Bv[b] + eB[b] * dt;
-                }
-            }
+                for (int f = 0; f < NF; ++f)
+                    if (f < nfc)
+                        fout[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
+                             ps.yy[k]] = Fo[f];
         }
         // advance the rings by one plane
 #pragma unroll
@@ -597,6 +685,257 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
                 for (int f = 0; f < NF; ++f) F[r][k][f] = F[r + 1][k][f];
     }
 }
+
+// ---------------------------------------------------------------------------
+// Two steps in one pass (temporal blocking).  The workgroup marches planes as
+// k_sweep does; at plane x it computes step k+1 on plane x (sub-step A, from
+// the step-k ring and the currents the channel kernel left in sJ) and step
+// k+2 on plane x-1 (sub-step B, from the ring of step-k+1 planes x-2, x-1, x
+// that A produced).  Only step k+2 is written: one read and one write of the
+// state per two steps.  The currents of step k+2 are those the channel kernel
+// would compute from the step-k+1 mouth values and the states it gated at
+// step k+1 (already in state[]): J = state * g * (jSR - c_mouth), the last
+// channel on a node winning; the mouth values of step k+1 (ions, bound
+// fields, free buffers) are written to mouths[] for the next channel call,
+// which gates step k+2 from them.  Rim nodes stay at their step-k values
+// (constant Dirichlet rim).  Chunks of one site recompute one halo plane of
+// step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
+// two steps here equal two k_sweep steps bit for bit.
+template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL>
+__global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
+                                                const T *__restrict__ fin, T *__restrict__ fout,
+                                                const int *__restrict__ ch_off, const int *__restrict__ snode,
+                                                const int *__restrict__ sperm, const double *__restrict__ sJ,
+                                                const int *__restrict__ cstate, const int *__restrict__ ctype,
+                                                const double *__restrict__ jsr, double *__restrict__ mouths,
+                                                const Channels a, Params p, double dt_, int chunk, int nchunks,
+                                                int units) {
+    constexpr int LI = NI ? NI : MAXI;
+    constexpr int LB = NB ? NB : MAXB;
+    constexpr int NG = NGT ? NGT : LI + LB;
+    constexpr int NF = NGT ? (NFT ? NFT : 1) : LB;
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int per = gridDim.x >> 3;
+    const int unit = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (unit >= units) return;
+    const int s = unit / nchunks;
+    const int cidx = unit - s * nchunks;
+    const int x0 = 1 + cidx * chunk;
+    const int x1 = min(p.nx - 1, x0 + chunk);  // step k+2 planes [x0, x1)
+    const int xa0 = x0 - 1;                    // step k+1 planes [xa0, x1]
+    const int P = p.ny * p.nz;
+    const int ng = p.n_grid, nfc = p.n_face;
+    const size_t fs = (size_t)p.S * p.NS;
+    const T *gs = gin + (size_t)s * p.NS;
+    T *os = gout + (size_t)s * p.NS;
+    const T dt = (T)dt_;
+    const T ihx2 = (T)p.ihx2, ihy2 = (T)p.ihy2, ihz2 = (T)p.ihz2;
+    T *LA = lds;                       // step-k planes, double-buffered
+    T *LB2 = lds + (size_t)2 * ng * P;  // step-k+1 planes, double-buffered
+
+    PlaneSlots<DIM, SL> ps;
+    ps.init(p, P);
+
+    T R[4][SL][NG];   // step k: planes x-1, x, x+1, x+2 (prefetch)
+    T F[2][SL][NF];   // step k faces: planes x, x+1 (prefetch)
+    T A[3][SL][NG];   // step k+1: planes x-2, x-1, x
+    T FA[2][SL][NF];  // step k+1 faces: planes x-1, x
+    auto load_plane = [&](T (&D)[SL][NG], int x) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng && ps.act[k]) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
+    };
+    auto load_face = [&](T (&D)[SL][NF], int x) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f < nfc && ps.fz[k] >= 0)
+                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
+                                  ps.yy[k]];
+    };
+    // planes xa0-1 .. xa0+1 (plane -1 does not exist: A copies the rim plane 0)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int xr = xa0 - 1 + r;
+        if (xr >= 0 && xr <= p.nx - 1) load_plane(R[r], xr);
+    }
+    load_face(F[0], xa0);
+
+    const int chi = ch_off[s + 1];
+    int curA = first_channel_at(snode, ch_off[s], chi, xa0 * P);
+    int curB = first_channel_at(snode, ch_off[s], chi, x0 * P);
+    const int off = ch_off[s];
+
+    for (int x = xa0; x <= x1; ++x) {
+        if (x + 2 <= min(x1 + 1, p.nx - 1)) load_plane(R[3], x + 2);
+        if (x + 1 <= x1) load_face(F[1], x + 1);
+        const bool doB = x - 1 >= x0;
+        const bool interiorA = x >= 1 && x <= p.nx - 2;
+        T *La = LA + (size_t)(x & 1) * ng * P;
+        T *Lb = LB2 + (size_t)((x - 1) & 1) * ng * P;
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng && ps.act[k]) {
+                    if (interiorA) La[g * P + ps.q[k]] = R[1][k][g];
+                    if (doB) Lb[g * P + ps.q[k]] = A[1][k][g];
+                }
+        lds_barrier();
+
+        // sub-step A: step k+1 on plane x
+        {
+            T J[SL][LI];
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+#pragma unroll
+                for (int j = 0; j < LI; ++j) J[k][j] = 0;
+            const int pend = (x + 1) * P;
+            while (curA < chi) {
+                const int nd = snode[curA];
+                if (nd >= pend) break;
+                const int qq = nd - x * P;
+#pragma unroll
+                for (int k = 0; k < SL; ++k)
+                    if (ps.q[k] == qq)
+#pragma unroll
+                        for (int j = 0; j < LI; ++j)
+                            if (NI || j < p.ni) J[k][j] = (T)sJ[(size_t)j * a.nch + curA];
+                ++curA;
+            }
+#pragma unroll
+            for (int k = 0; k < SL; ++k) {
+                if (interiorA && ps.inner[k]) {
+                    const int c = ps.q[k];
+                    const int z = DIM == 3 ? c % p.nz : 0;
+                    node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], La, c, P, ps.zl[k], ps.zr[k],
+                                                        z, ps.fz[k] >= 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2,
+                                                        A[2][k], FA[1][k]);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) A[2][k][g] = R[1][k][g];
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) FA[1][k][f] = F[0][k][f];
+                }
+            }
+        }
+
+        // sub-step B: step k+2 on plane x-1
+        if (doB) {
+            const int xb = x - 1;
+            T J[SL][LI];
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+#pragma unroll
+                for (int j = 0; j < LI; ++j) J[k][j] = 0;
+            const int pend = (xb + 1) * P;
+            while (curB < chi) {
+                const int nd = snode[curB];
+                if (nd >= pend) break;
+                const int qq = nd - xb * P;
+                const int i = sperm[curB];
+#pragma unroll
+                for (int k = 0; k < SL; ++k)
+                    if (ps.q[k] == qq) {
+                        // the channel kernel's snapshot and current (k_channels), step k+1 values
+                        const int st = cstate[off + i];
+                        const int typ = ctype[off + i];
+                        const int z = DIM == 3 ? qq % p.nz : 0;
+                        const bool fc = DIM == 3 && (z == 0 || z == p.nz - 1);
+                        double bnd[LB], fre[MAXU];
+#pragma unroll
+                        for (int b = 0; b < LB; ++b) {
+                            if (!NB && b >= p.nb) break;
+                            double v = 0.0;
+                            const int g = ly_bslot<LY>(p, b);
+                            if (ly_bface<LY>(p, b)) {
+#pragma unroll
+                                for (int f = 0; f < NF; ++f)
+                                    if (f == g && fc) v = (double)FA[0][k][f];
+                            } else {
+#pragma unroll
+                                for (int gg = 0; gg < NG; ++gg)
+                                    if (gg == g) v = (double)A[1][k][gg];
+                            }
+                            bnd[b] = v;
+                        }
+                        // free buffers: total minus bound, subtracted in bound order
+#pragma unroll
+                        for (int u = 0; u < Layout<LY>::nu; ++u) {
+                            double v = u < p.nu ? ly_total<LY>(p, u, z) : 0.0;
+#pragma unroll
+                            for (int b = 0; b < LB; ++b) {
+                                if (!NB && b >= p.nb) break;
+                                if (ly_bbuf<LY>(p, b) == u) v -= bnd[b];
+                            }
+                            fre[u] = v;
+                        }
+                        double *mo = mouths + i;
+                        const size_t MS = (size_t)a.M * p.S;
+#pragma unroll
+                        for (int j = 0; j < LI; ++j) {
+                            if (!NI && j >= p.ni) break;
+                            const double ion = (double)A[1][k][j];
+                            mo[(size_t)a.M * s + j * MS] = ion;
+                            const double jv = jsr[i + (size_t)a.M * (s + (size_t)j * p.S)];
+                            const double cond = a.g[typ * MAXI + j] * (jv - ion);
+                            J[k][j] = (T)((double)st * cond);
+                        }
+#pragma unroll
+                        for (int b = 0; b < LB; ++b)
+                            if (b < p.nb) mo[(size_t)a.M * s + (p.ni + b) * MS] = bnd[b];
+#pragma unroll
+                        for (int u = 0; u < Layout<LY>::nu; ++u)
+                            if (u < p.nu) mo[(size_t)a.M * s + (p.ni + p.nb + u) * MS] = fre[u];
+                    }
+                ++curB;
+            }
+#pragma unroll
+            for (int k = 0; k < SL; ++k) {
+                if (!ps.inner[k]) continue;
+                const int c = ps.q[k];
+                const int z = DIM == 3 ? c % p.nz : 0;
+                T G[NG], Fo[NF];
+                node_update<T, DIM, NI, NB, NG, NF, LY>(p, A[0][k], A[1][k], A[2][k], Lb, c, P, ps.zl[k], ps.zr[k], z,
+                                                    ps.fz[k] >= 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
+                const size_t o = (size_t)xb * P + c;
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    if (g < ng) os[g * fs + o] = G[g];
+                if (ps.fz[k] >= 0)
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        if (f < nfc)
+                            fout[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)xb * p.ny +
+                                 ps.yy[k]] = Fo[f];
+            }
+        }
+        // advance the rings by one plane
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                R[0][k][g] = R[1][k][g];
+                R[1][k][g] = R[2][k][g];
+                R[2][k][g] = R[3][k][g];
+                A[0][k][g] = A[1][k][g];
+                A[1][k][g] = A[2][k][g];
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                F[0][k][f] = F[1][k][f];
+                FA[0][k][f] = FA[1][k][f];
+            }
+        }
+    }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Kinetic rates (SheepCannell.cpp:12-18, TwoStateModel.cpp:4-8,
@@ -695,7 +1034,17 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
         int st = valid ? a.state[off + i] : 0;
         const int typ = valid ? a.type[off + i] : 0;
         double ion[MAXI], bnd[MAXB], fre[MAXU];
-        if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE)) {
+        if (mode & CH_FROM_STORED) {
+            // the snapshot k_sweep2 wrote for these channels
+            const size_t MS = (size_t)a.M * p.S, base = (size_t)i + (size_t)a.M * s;
+#pragma unroll
+            for (int j = 0; j < MAXI; ++j) ion[j] = (j < p.ni && valid) ? a.mouths[base + j * MS] : 0.0;
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) bnd[b] = (b < p.nb && valid) ? a.mouths[base + (p.ni + b) * MS] : 0.0;
+#pragma unroll
+            for (int u = 0; u < MAXU; ++u)
+                fre[u] = (u < p.nu && valid) ? a.mouths[base + (p.ni + p.nb + u) * MS] : 0.0;
+        } else if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE)) {
             const int node = valid ? a.node[off + i] : 0;
             const int z = p.dim == 3 ? node % p.nz : 0;
             const int xy = p.dim == 3 ? node / p.nz : node;

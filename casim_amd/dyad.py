@@ -174,5 +174,11 @@ class DyadHIP:
     def SetTuning(self, planes=0, threads=0, prefetch=0):
         abi.check(self.lib.casim_dyad_set_tuning(self.h, planes, threads, prefetch))
 
+    def SetStepsPerPass(self, steps):
+        abi.check(self.lib.casim_dyad_set_steps_per_pass(self.h, steps))
+
+    def StepsPerPass(self):
+        return self.lib.casim_dyad_steps_per_pass(self.h)
+
     def stream(self):
         return self.lib.casim_dyad_stream(self.h)

@@ -309,6 +309,14 @@ double casim_dyad_bytes_per_site_step(const CasimDyad *d);
  * the tuning. */
 int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block, int32_t prefetch);
 
+/* Steps per state pass of casim_dyad_step: 2 (the default) advances pairs of
+ * steps in one read and one write of the state, where the configuration
+ * allows it (plane-marching kernel, no pools attached); 1 forces one pass per
+ * step.  Results do not depend on it.  casim_dyad_steps_per_pass returns the
+ * value in effect (1 or 2), -1 for a null handle. */
+int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps);
+int32_t casim_dyad_steps_per_pass(const CasimDyad *d);
+
 /* HIP stream of the handle (hipStream_t), for callers that time with events. */
 void *casim_dyad_stream(CasimDyad *d);
 

@@ -320,3 +320,49 @@ def test_pools_reference_test_input_cytosol(gpu, oracle_mod):
     assert rel_err(cbd, cbo) < FP64_TOL
     assert ora.jsr()[0].min() < 1000.0 and cio[0].max() > 0.1
     compare(ora.dyad, dev, sp, FP64_TOL)
+
+
+@pytest.mark.parametrize("precision,planes,threads", [(abi.CASIM_FP64, 0, 0), (abi.CASIM_FP64, 10, 256),
+                                                      (abi.CASIM_FP32, 0, 448), (abi.CASIM_FP32, 20, 192)])
+def test_two_step_passes_equal_single_steps(gpu, precision, planes, threads):
+    """Pairs of steps in one state pass give the single-step results bit for
+    bit: fields, channel histories, RNG, mouth snapshots, currents, effluxes
+    and release statistics, with fast gating so that channels toggle inside
+    passes, over odd and even step counts."""
+    say("two-step passes", precision, planes, threads)
+    sp = workloads.batched_dyads(n_sites=12, geometry_seed=9, precision=precision)
+    sp.channel_types[0].params["kopen0"] *= 1e6
+    sp.channel_types[0].params["kclose0"] *= 1e4
+    ri, rb = sp.rim_arrays()
+    j = sp.jsr_array([1000.0])
+    devs = []
+    for steps in (1, 2):
+        d = DyadHIP(sp)
+        if planes or threads:
+            d.SetTuning(planes, threads, 1)
+        d.SetStepsPerPass(steps)
+        d.InitOpeningAll()
+        d.Update(ri, rb, j)
+        for k in (7, 50, 1, 64):
+            d.Step(1e-5, k)
+        devs.append(d)
+    one, two = devs
+    assert one.StepsPerPass() == 1 and two.StepsPerPass() == 2
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(one.GetField(f), two.GetField(f)), f
+    st = one.GetChannelsStates()
+    assert np.array_equal(st, two.GetChannelsStates())
+    assert (st != workloads.batched_dyads(n_sites=12, geometry_seed=9).init_channel[0]).any()
+    assert np.array_equal(one.GetRngStates(), two.GetRngStates())
+    assert np.array_equal(one.GetIonsNearChannels(), two.GetIonsNearChannels())
+    assert np.array_equal(one.GetTotalSRCurrent(), two.GetTotalSRCurrent())
+    assert np.array_equal(one.GetEffluxes(), two.GetEffluxes())
+    assert np.array_equal(one.GetReleaseStats(), two.GetReleaseStats())
+
+
+def test_pools_run_one_step_per_pass(gpu):
+    sp, j = config.load_json(os.path.join(GOLDEN, "test_input_Parameters.json"), n_sites=2)
+    dev = DyadHIP(sp)
+    assert dev.StepsPerPass() == 2
+    dev.AttachPools(config.pools_from_json(j, sp))
+    assert dev.StepsPerPass() == 1
