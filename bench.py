@@ -29,7 +29,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False):
+def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False, tuning=None):
     from casim_amd import autotune
     sp = workloads.batched_dyads(n_sites=n_sites, precision=precision)
     dev = DyadHIP(sp)
@@ -38,6 +38,9 @@ def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, re
     dev.Update(ri, rb, sp.jsr_array([sp.meta["jsr_uM"]]))
     dt = sp.meta["dt_ms"]
     tuned = None if retune else autotune.apply(dev, sp)
+    if tuning:
+        dev.SetTuning(tuning[0], tuning[1], 1)
+        tuned = {"planes": tuning[0], "threads": tuning[1], "sweep_ms": None}
     if tuned is None:
         tuned, _ = autotune.tune(dev, sp, dt, log=log)
         # the tuning steps advanced the state: start the timed run from a reset
@@ -132,6 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--tuning", default=None, help="planes,threads of the state sweep (skips the autotuner)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: sites are independent and this build has no multi-GPU path "
@@ -141,12 +145,14 @@ def main():
     from casim_amd.dyad import DyadHIP
     abi.load()
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
-    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune)
+    tuning = [int(v) for v in a.tuning.split(",")] if a.tuning else None
+    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning)
     log("fp64:", json.dumps(r64))
     r32 = None
     if not a.no_fp32:
         log("fp32 leg")
-        r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune)
+        r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune,
+                          tuning)
         log("fp32:", json.dumps(r32))
     cpu = None
     if not a.no_cpu:
