@@ -89,7 +89,7 @@ def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, re
     return res
 
 
-def cpu_baseline(workloads, n_sites=32, steps=300):
+def cpu_baseline(workloads, n_sites=32, steps=5000):
     """The oracle (C restatement of DyadRDCPU, OpenMP over sites as
     ParallelHybridSolver does) on the first n_sites sites of the same batch."""
     from oracle import oracle as O
@@ -110,6 +110,28 @@ def cpu_baseline(workloads, n_sites=32, steps=300):
     t = time.perf_counter() - t0
     return {"value": n_sites * steps * dt / t, "unit": "dyad-ms/s", "cores": threads, "kind": "port",
             "sample": "first %d sites of the same 4096-site batch, %d steps (%.1f s wall)" % (n_sites, steps, t)}
+
+
+def surrogate_leg(workloads, n_sites, t_max, plain_value):
+    """The surrogate-driven quasi-stationary path (the event-driven release
+    site with the learned mouth-[Ca2+] ensemble) on the same sites and
+    kinetics as the plain leg: every site from rest, its init channel open,
+    to t_max, jSR constant.  Throughput in the plain leg's unit."""
+    w = os.path.join(ROOT, "casim_amd", "surrogate_weights.npz")
+    if not os.path.exists(w):
+        return None
+    from casim_amd import mlp, spark_compare
+    net = mlp.MLP.load(w)
+    spec = workloads.batched_dyads(n_sites=n_sites)
+    spark_compare.run_qs(spec, net, 0.05)  # warm-up: workspaces and kernels loaded
+    win, full, wall, fb = spark_compare.run_qs(spec, net, t_max)
+    meta = mlp.MLP.meta(w)
+    held = meta.get("heldout", {})
+    value = n_sites * t_max / wall
+    return {"value": value, "unit": "dyad-ms/s", "speedup_vs_plain": value / plain_value, "t_max_ms": t_max,
+            "wall_s": wall, "events_per_site": float(full[:, 5].mean()), "fallback_sites": int(fb),
+            "mouth_ca_rel_err_heldout": {k: held.get(k) for k in ("rel_mean", "rel_p50", "rel_p95", "rel_mean_above_5uM")},
+            "target": meta.get("target")}
 
 
 def traffic_from_profiles(kernel):
@@ -136,6 +158,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--tuning", default=None, help="planes,threads of the state sweep (skips the autotuner)")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--qs-tmax", type=float, default=20.0, help="simulated ms of the surrogate leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: sites are independent and this build has no multi-GPU path "
@@ -154,6 +178,11 @@ def main():
         r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune,
                           tuning)
         log("fp32:", json.dumps(r32))
+    sur = None
+    if not a.no_surrogate:
+        log("surrogate leg")
+        sur = surrogate_leg(workloads, a.sites, a.qs_tmax, r64["dyad_ms_per_s"])
+        log("surrogate:", json.dumps(sur))
     cpu = None
     if not a.no_cpu:
         log("cpu baseline")
@@ -189,6 +218,7 @@ def main():
             "value": r32["dyad_ms_per_s"], "ms_per_step": r32["ms_per_step"], "sweep_gbs": r32["sweep_gbs"],
             "sweep_kernel_ms": r32["sweep_kernel_ms"]},
         "kernels": {"sweep_ms": r64["sweep_kernel_ms"], "channels_ms": r64["channel_kernel_ms"]},
+        "surrogate": sur,
         "finite": r64["finite"] and (r32 is None or r32["finite"]),
     }
     print(json.dumps(out), flush=True)

@@ -158,13 +158,12 @@ void CasimDyad::launch_rim(const double *ions, const double *bound) {
 }
 
 void CasimDyad::launch_efflux() {
-    const int n = P.S * (P.ni + P.nb);
-    dim3 grid((n + 127) / 128);
+    dim3 grid(P.S * (P.ni + P.nb));
     if (precision == CASIM_FP64)
-        hipLaunchKernelGGL((k_efflux<double>), grid, dim3(128), 0, stream, (const double *)g[cur], d_eff, P,
+        hipLaunchKernelGGL((k_efflux<double>), grid, dim3(64), 0, stream, (const double *)g[cur], d_eff, P,
                            grad_mult, sxm, sym);
     else
-        hipLaunchKernelGGL((k_efflux<float>), grid, dim3(128), 0, stream, (const float *)g[cur], d_eff, P,
+        hipLaunchKernelGGL((k_efflux<float>), grid, dim3(64), 0, stream, (const float *)g[cur], d_eff, P,
                            grad_mult, sxm, sym);
     HIPCK(hipGetLastError());
 }

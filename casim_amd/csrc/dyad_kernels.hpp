@@ -1244,16 +1244,19 @@ __global__ void k_rim(T *g0, T *g1, T *f0, T *f1, const double *__restrict__ rim
     }
 }
 
-// _GetGradients (DyadRDCPU.cpp:322-369, DyadRD2DCPU.cpp:305-350): one thread
-// per (site, field), terms summed in the reference's loop order.
+// _GetGradients (DyadRDCPU.cpp:322-369, DyadRD2DCPU.cpp:305-350): the same
+// one-sided rim terms, one wave per (site, field).  The sum is regrouped (lane
+// partial sums, then a shuffle tree), so it matches the reference's serial sum
+// to rounding, and is identical from call to call.
 template <typename T>
-__global__ void k_efflux(const T *__restrict__ g, double *__restrict__ out, Params p, double gm, double sxm,
-                         double sym) {
+__global__ __launch_bounds__(64) void k_efflux(const T *__restrict__ g, double *__restrict__ out, Params p, double gm,
+                                               double sxm, double sym) {
 #pragma clang fp contract(off)
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per (site, field): lanes take rim terms in turn, then a fixed
+    // shuffle tree adds the lane sums (same order on every call)
     const int nf = p.ni + p.nb;
-    if (tid >= p.S * nf) return;
-    const int s = tid / nf, fi = tid % nf;
+    const int s = blockIdx.x / nf, fi = blockIdx.x - (blockIdx.x / nf) * nf;
+    const int lane = threadIdx.x;
     double D;
     const T *u;
     const size_t fs = (size_t)p.S * p.NS;
@@ -1264,34 +1267,42 @@ __global__ void k_efflux(const T *__restrict__ g, double *__restrict__ out, Para
         const int b = fi - p.ni;
         D = p.Db[b];
         if (p.bface[b]) {
-            out[fi + nf * s] = 0.0;  // membrane buffers do not diffuse
+            if (lane == 0) out[fi + nf * s] = 0.0;  // membrane buffers do not diffuse
             return;
         }
         u = g + p.bslot[b] * fs + (size_t)s * p.NS;
     }
     const int zmax = p.dim == 3 ? p.nz - 1 : 1;
     const int nz = p.nz, ny = p.ny, nx = p.nx;
-    double acc = 0.0;
     const double tx = D * gm * sxm, ty = D * gm * sym;
-    for (int x = 0; x < nx; x += nx - 1)
-        for (int y = 1; y < ny - 1; ++y)
-            for (int z = 0; z < zmax; ++z) {
-                int d = x == 0 ? 1 : -1;
-                double u1 = (double)u[z + nz * (y + ny * (x + d))];
-                double u0 = (double)u[z + nz * (y + ny * x)];
-                double u2 = (double)u[z + nz * (y + ny * (x + 2 * d))];
-                acc += tx * (2 * u1 - 1.5 * u0 - 0.5 * u2);
-            }
-    for (int y = 0; y < ny; y += ny - 1)
-        for (int x = 0; x < nx; ++x)
-            for (int z = 0; z < zmax; ++z) {
-                int d = y == 0 ? 1 : -1;
-                double u1 = (double)u[z + nz * (y + d + ny * x)];
-                double u0 = (double)u[z + nz * (y + ny * x)];
-                double u2 = (double)u[z + nz * (y + 2 * d + ny * x)];
-                acc += ty * (2 * u1 - 1.5 * u0 - 0.5 * u2);
-            }
-    out[fi + nf * s] = acc;
+    double acc = 0.0;
+    // x-rims: x = 0 and nx-1, y in [1, ny-2], z in [0, zmax)
+    const int per_x = (ny - 2) * zmax;
+    for (int t = lane; t < 2 * per_x; t += 64) {
+        const int side = t >= per_x;
+        const int r = t - side * per_x;
+        const int y = 1 + r / zmax, z = r - (r / zmax) * zmax;
+        const int x = side ? nx - 1 : 0, d = side ? -1 : 1;
+        const double u1 = (double)u[z + nz * (y + ny * (x + d))];
+        const double u0 = (double)u[z + nz * (y + ny * x)];
+        const double u2 = (double)u[z + nz * (y + ny * (x + 2 * d))];
+        acc += tx * (2 * u1 - 1.5 * u0 - 0.5 * u2);
+    }
+    // y-rims: y = 0 and ny-1, x in [0, nx), z in [0, zmax)
+    const int per_y = nx * zmax;
+    for (int t = lane; t < 2 * per_y; t += 64) {
+        const int side = t >= per_y;
+        const int r = t - side * per_y;
+        const int x = r / zmax, z = r - (r / zmax) * zmax;
+        const int y = side ? ny - 1 : 0, d = side ? -1 : 1;
+        const double u1 = (double)u[z + nz * (y + d + ny * x)];
+        const double u0 = (double)u[z + nz * (y + ny * x)];
+        const double u2 = (double)u[z + nz * (y + 2 * d + ny * x)];
+        acc += ty * (2 * u1 - 1.5 * u0 - 0.5 * u2);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) out[fi + nf * s] = acc;
 }
 
 }  // namespace casim

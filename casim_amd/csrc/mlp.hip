@@ -214,6 +214,7 @@ struct CasimMLP {
     float *mean = nullptr, *istd = nullptr;
     float ym = 0.f, ys = 1.f;
     float *X = nullptr, *Y = nullptr;
+    int *perm_d = nullptr;  // [rows] shuffled row order of the current epoch
     int64_t rows = 0;
     // batch workspace
     int cap = 0;
@@ -421,7 +422,7 @@ int casim_mlp_destroy(CasimMLP *m) {
     for (auto v : {&m->p, &m->g, &m->am, &m->av})
         for (float *q : *v) (void)hipFree(q);
     for (uint16_t *q : m->pb) (void)hipFree(q);
-    for (void *q : {(void *)m->mean, (void *)m->istd, (void *)m->X, (void *)m->Y})
+    for (void *q : {(void *)m->mean, (void *)m->istd, (void *)m->X, (void *)m->Y, (void *)m->perm_d})
         if (q) (void)hipFree(q);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -474,9 +475,12 @@ int casim_mlp_set_data(CasimMLP *m, const float *x, const float *y, int64_t rows
         m->use();
         if (m->X) (void)hipFree(m->X);
         if (m->Y) (void)hipFree(m->Y);
+        if (m->perm_d) (void)hipFree(m->perm_d);
         m->X = m->Y = nullptr;
+        m->perm_d = nullptr;
         HIPCK(hipMalloc(&m->X, sizeof(float) * rows * m->n_in));
         HIPCK(hipMalloc(&m->Y, sizeof(float) * rows));
+        HIPCK(hipMalloc(&m->perm_d, sizeof(int) * rows));
         HIPCK(hipMemcpyAsync(m->X, x, sizeof(float) * rows * m->n_in, hipMemcpyHostToDevice, m->stream));
         HIPCK(hipMemcpyAsync(m->Y, y, sizeof(float) * rows, hipMemcpyHostToDevice, m->stream));
         HIPCK(hipStreamSynchronize(m->stream));
@@ -503,14 +507,16 @@ int casim_mlp_train(CasimMLP *m, int32_t epochs, int32_t batch, float lr, uint64
             for (int ep = 0; ep < epochs; ++ep) {
                 std::iota(perm.begin(), perm.end(), 0);
                 std::shuffle(perm.begin(), perm.end(), gen);
+                HIPCK(hipMemcpyAsync(m->perm_d, perm.data(), sizeof(int) * m->rows, hipMemcpyHostToDevice,
+                                     m->stream));
                 double sum = 0.0;
                 int64_t cnt = 0;
                 const bool keep = ep == epochs - 1;
                 for (int64_t r0 = 0; r0 + B <= m->rows; r0 += B) {
-                    HIPCK(hipMemcpyAsync(m->idx, perm.data() + r0, sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
                     const size_t n = (size_t)B * m->n_in;
                     hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->X,
-                                       m->Y, m->idx, B, m->n_in, m->mean, m->istd, m->ym, yis, m->Xl[0], m->tgt);
+                                       m->Y, (const int *)(m->perm_d + r0), B, m->n_in, m->mean, m->istd, m->ym, yis,
+                                       m->Xl[0], m->tgt);
                     HIPCK(hipGetLastError());
                     m->forward(e, B);
                     hipLaunchKernelGGL(k_mse, dim3((B + 255) / 256), dim3(256), 0, m->stream, m->Zl[m->depth], m->tgt,

@@ -17,7 +17,13 @@ constexpr int QS_NEAR = 16;
 constexpr int QS_F = 8 + 3 * QS_NEAR;  // features per RyR
 constexpr int QS_MAXCH = 128;
 
-enum QsStat { QS_AMP = 0, QS_TTP, QS_TTE, QS_Q, QS_TCUR, QS_NCUR, QS_EVENTS, QS_NSTAT };
+// Per-site statistics: the event record's columns (QS_AMP .. QS_EVENTS) and
+// the same quantities cut at t_max (QS_W*), for comparison with fixed-step runs.
+enum QsStat {
+    QS_AMP = 0, QS_TTP, QS_TTE, QS_Q, QS_TCUR, QS_NCUR, QS_EVENTS,
+    QS_WAMP, QS_WTTE, QS_WQ, QS_WN,
+    QS_NSTAT
+};
 
 struct QsDev {
     int S;
@@ -115,17 +121,18 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
 }
 
 // Ensemble mean / spread of every channel of active sites, then one Gillespie
-// event per site unless the spread asks for the full solver.
+// event per site unless the surrogate is too uncertain for it: the site asks
+// for the full solver when the rate-weighted mean of the relative spreads,
+// sum_i (rate_i / total) spread_i, exceeds the threshold.
 __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     __shared__ double rate[QS_MAXCH];
-    __shared__ float wmax[64];
+    __shared__ float spv[QS_MAXCH];
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
     const int base = q.crow[s];
     if (base < 0) return;
     const int off = q.off[s], n = q.off[s + 1] - off;
     const int wait = q.waiting[s];
-    float smax = 0.f;
     for (int i = lane; i < n; i += 64) {
         const size_t row = off + i;
         float ca, sp;
@@ -145,30 +152,34 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
         }
         q.ca[row] = ca;
         q.spread[row] = sp;
-        smax = fmaxf(smax, sp);
+        spv[i] = sp;
         const int st = q.state[row];
         rate[i] = channel_rate(q.model, q.par, st, (double)ca, q.mg);
     }
-    wmax[lane] = smax;
     __syncthreads();
     if (lane != 0) return;
-    float m = 0.f;
-    for (int k = 0; k < 64; ++k) m = fmaxf(m, wmax[k]);
-    if (wait == 0 && m > q.thresh) {
-        q.waiting[s] = 1;
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) total += rate[i];
+    double *st = q.stats + (size_t)s * QS_NSTAT;
+    if (!(total > 0.0)) {
+        // nothing can happen any more: N stays until t_max
+        st[QS_WQ] += st[QS_NCUR] * (q.t_max - fmin(st[QS_TCUR], q.t_max));
+        q.active[s] = 0;
+        q.waiting[s] = 0;
         return;
+    }
+    if (wait == 0) {
+        double m = 0.0;
+        for (int i = 0; i < n; ++i) m += rate[i] / total * (double)spv[i];
+        if (m > (double)q.thresh) {
+            q.waiting[s] = 1;
+            return;
+        }
     }
     q.waiting[s] = 0;
     // Gillespie step (QuasyStationaryReleaseSite.cpp:101-128)
     uint32_t S6[6];
     for (int k = 0; k < 6; ++k) S6[k] = q.rng[s * 6 + k];
-    double total = 0.0;
-    for (int i = 0; i < n; ++i) total += rate[i];
-    double *st = q.stats + (size_t)s * QS_NSTAT;
-    if (!(total > 0.0)) {
-        q.active[s] = 0;
-        return;
-    }
     const double u1 = mrg::draw(S6);
     const double dtw = -log(u1) * (1.0 / total);
     const double t = q.t[s] + dtw;
@@ -192,6 +203,13 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     if (nnew > st[QS_AMP]) {
         st[QS_AMP] = nnew;
         st[QS_TTP] = t;
+    }
+    // the window [0, t_max]
+    st[QS_WQ] += ncur * (fmin(t, q.t_max) - fmin(st[QS_TCUR], q.t_max));
+    if (t <= q.t_max) {
+        st[QS_WTTE] = t;
+        st[QS_WN] = nnew;
+        if (nnew > st[QS_WAMP]) st[QS_WAMP] = nnew;
     }
     st[QS_TCUR] = t;
     st[QS_TTE] = t;
@@ -222,6 +240,8 @@ __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t 
     for (int k = 0; k < QS_NSTAT; ++k) st[k] = 0.0;
     st[QS_AMP] = 1.0;
     st[QS_NCUR] = 1.0;
+    st[QS_WAMP] = 1.0;
+    st[QS_WN] = 1.0;
 }
 
 // Rows of this round: every site with `all`, else the active sites that do
@@ -512,6 +532,25 @@ int casim_qs_get_stats(CasimQS *h, double *out) {
             o[3] = a[QS_Q];
             o[4] = a[QS_TTE] > 0 ? a[QS_Q] / a[QS_TTE] : 0.0;
             o[5] = a[QS_EVENTS];
+        }
+    });
+}
+
+int casim_qs_get_window_stats(CasimQS *h, double *out) {
+    return guard([&] {
+        REQUIRE(h && out, CASIM_E_INVALID, "null argument");
+        h->use();
+        const int S = h->q.S;
+        std::vector<double> st((size_t)S * QS_NSTAT);
+        HIPCK(hipMemcpyAsync(st.data(), h->q.stats, sizeof(double) * st.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+        for (int s = 0; s < S; ++s) {
+            const double *a = &st[(size_t)s * QS_NSTAT];
+            double *o = out + (size_t)s * 4;
+            o[0] = a[QS_WAMP];
+            o[1] = a[QS_WTTE];
+            o[2] = a[QS_WQ];
+            o[3] = a[QS_WN];
         }
     });
 }

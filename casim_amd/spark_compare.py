@@ -1,0 +1,142 @@
+"""Release statistics and throughput of the plain path (full reaction-diffusion
+with per-step gating) against the surrogate-driven quasi-stationary path on
+the same sites.
+
+    python -m casim_amd.spark_compare --sites 1024 --t-max 2 --weights casim_amd/surrogate_weights.npz
+
+Both runs start every site from rest with its init channel open, with a
+constant jSR; statistics are those of IndChannelCREAnalyzer over [0, t_max]
+(IndChannelCREAnalyzer.cpp:69-117): amplitude, time of the last event,
+integral of N dt, and N at t_max.  The two populations are compared with
+two-sample KS tests.  Progress and results are JSON lines.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import abi, analysis, config, workloads
+
+# SheepCannell parameters of the reference's own example and test inputs
+# (test_input/Parameters.json, examples/CannellCPU/Parameters.json)
+REFERENCE_RYR = {"kopen0": 457000.0, "ncaopen": 2.12, "kopenmax": 800.0, "kclose0": 245.0, "ncaclose": -0.27}
+
+
+def workload(n_sites, geometry_seed=2024, kinetics="reference", precision=abi.CASIM_FP64):
+    spec = workloads.batched_dyads(n_sites=n_sites, geometry_seed=geometry_seed, precision=precision)
+    if kinetics == "reference":
+        spec.channel_types = [config.ChannelType("RyR", "SheepCannell", dict(REFERENCE_RYR),
+                                                 [workloads.unitary_conductance()])]
+    return spec
+
+
+def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None):
+    """Full RD + gating for t_max ms: release stats [S, 6], wall seconds."""
+    from .dyad import DyadHIP
+    dt = workloads.DT_MS
+    steps = int(round(t_max / dt))
+    dev = DyadHIP(spec)
+    dev.InitOpeningAll()
+    ri, rb = spec.rim_arrays()
+    dev.Update(ri, rb, spec.jsr_array([jsr]))
+    dev.Step(dt, min(chunk, steps))  # graph capture outside the timed region
+    dev.Reset()
+    dev.InitOpeningAll()
+    dev.Update(ri, rb, spec.jsr_array([jsr]))
+    dev.Synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        dev.Step(dt, k)
+        done += k
+        if log and done % (20 * chunk) == 0:
+            dev.Synchronize()
+            log({"stage": "plain", "steps": done, "of": steps, "s": round(time.perf_counter() - t0, 1)})
+    dev.Synchronize()
+    wall = time.perf_counter() - t0
+    stats = dev.GetReleaseStats()
+    dev.close()
+    return stats, wall
+
+
+def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None):
+    """Surrogate-driven quasi-stationary path to t_max: window stats [S, 4],
+    event-record stats [S, 6], wall seconds, fallback solves."""
+    from .surrogate import QuasiStationarySites
+    qs = QuasiStationarySites(spec, np.full(spec.n_sites, jsr), t_max, net, spread_threshold=threshold)
+    t0 = time.perf_counter()
+    qs.RunSimulation()
+    wall = time.perf_counter() - t0
+    win, full = qs.GetWindowStats(), qs.GetStats()
+    fb = qs.fallback_solves
+    qs.close()
+    if log:
+        log({"stage": "qs", "s": round(wall, 2), "events": float(full[:, 5].sum()), "fallback_sites": fb})
+    return win, full, wall, fb
+
+
+def on_grid(t, dt):
+    """Times and time integrals at the fixed-step path's resolution: the plain
+    path sees events only at multiples of dt and sums N dt over steps, so a
+    continuous-time value is rounded to the nearest multiple of dt (without
+    this, the point mass of sites open through the whole window, exactly
+    t_max in one path and a 10^5-term FP sum in the other, separates the
+    samples)."""
+    return np.round(np.asarray(t) / dt) * dt
+
+
+def compare(plain, qs_win, n_channels, t_max, alpha=0.01, dt=workloads.DT_MS):
+    """KS tests of amplitude, last event time, integral of N dt and N at t_max."""
+    cols = {"amplitude": (plain[:, 0], qs_win[:, 0]),
+            "last_event_ms": (on_grid(plain[:, 2], dt), on_grid(qs_win[:, 1], dt)),
+            "open_integral": (on_grid(plain[:, 3], dt), on_grid(qs_win[:, 2], dt)),
+            "n_open_end": (plain[:, 5], qs_win[:, 3])}
+    out = {}
+    for name, (a, b) in cols.items():
+        d, p = analysis.ks_2samp(a, b)
+        out[name] = {"D": d, "p": p, "pass": p > alpha, "mean_plain": float(np.mean(a)), "mean_qs": float(np.mean(b))}
+    pa = analysis.Population(np.column_stack([plain[:, 0], plain[:, 1], plain[:, 2], plain[:, 3], plain[:, 4],
+                                              plain[:, 5]]), n_channels, t_max)
+    out["classes_plain"] = pa.classes()
+    qsfull = np.zeros((len(qs_win), 6))
+    qsfull[:, 0] = qs_win[:, 0]
+    out["classes_qs"] = analysis.Population(qsfull, n_channels, t_max).classes()
+    out["all_pass"] = all(v["pass"] for k, v in out.items() if isinstance(v, dict) and "pass" in v)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sites", type=int, default=1024)
+    ap.add_argument("--t-max", type=float, default=2.0)
+    ap.add_argument("--kinetics", default="reference", choices=["reference", "bench"])
+    ap.add_argument("--weights", default="casim_amd/surrogate_weights.npz")
+    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--jsr", type=float, default=1000.0)
+    ap.add_argument("--skip-plain", action="store_true")
+    a = ap.parse_args(argv)
+
+    def log(d):
+        print(json.dumps(d), flush=True)
+
+    from . import mlp
+    spec = workload(a.sites, kinetics=a.kinetics)
+    n_ch = np.array([len(s) for s in spec.sites])
+    net = mlp.MLP.load(a.weights)
+    win, full, wq, fb = run_qs(spec, net, a.t_max, a.jsr, a.threshold, log)
+    res = {"stage": "result", "sites": a.sites, "t_max_ms": a.t_max, "kinetics": a.kinetics,
+           "qs_wall_s": wq, "qs_dyad_ms_per_s": a.sites * a.t_max / wq, "fallback_sites": fb,
+           "events_per_site": float(full[:, 5].mean())}
+    if not a.skip_plain:
+        plain, wp = run_plain(spec, a.t_max, a.jsr, log=log)
+        res.update({"plain_wall_s": wp, "plain_dyad_ms_per_s": a.sites * a.t_max / wp,
+                    "speedup": wp / wq, "ks": compare(plain, win, n_ch, a.t_max)})
+    log(res)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

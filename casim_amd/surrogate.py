@@ -3,9 +3,12 @@ features, and the quasi-stationary release-site path it drives.
 
 Target.  For a site geometry, a clamped open set and a jSR concentration, the
 target of every RyR is ln [Ca2+] at its mouth after a fixed relaxation of the
-full reaction-diffusion solver from rest (QS_STEPS steps of dt = 10 ns): the
-fast quasi-equilibrium of free Ca2+ and the ATP-like buffer that gating sees
-after an event.  The same solve is the ensemble's fallback.
+full reaction-diffusion solver from rest (QS_STEPS steps of dt = 10 ns, 0.5
+ms): free Ca2+, the fast ATP-like buffer and most of the slower buffer
+loading have settled, which is the quasi-stationary value gating sees while
+an open set persists (QuasyStationaryReleaseSite takes a stationary
+solution, StationaryDiffusionModel.cpp:3-11).  The same solve is the
+ensemble's fallback.
 
 Features of RyR i (lattice units of 31 nm, cleft centre at the origin):
   0  open flag of i            1  jSR [Ca2+] / 1000 uM     2  open RyRs / 100
@@ -24,7 +27,7 @@ from . import abi, config, mlp, workloads
 PITCH_NM = 31.0
 N_NEAR = 16
 N_FEAT = 8 + 3 * N_NEAR  # 56
-QS_STEPS = 2000
+QS_STEPS = 50_000
 DT_MS = 1e-5
 
 
@@ -117,7 +120,7 @@ def qs_solve(DyadHIP, spec, states, jsr_site, steps=QS_STEPS, tuning=None):
     dev.Update(ri, rb, jsr_layout(spec, jsr_site))
     done = 0
     while done < steps:
-        k = min(500, steps - done)
+        k = min(1000, steps - done)
         dev.Step(DT_MS, k)
         done += k
     m = dev.GetIonsNearChannels()[0].copy()
@@ -231,6 +234,12 @@ class QuasiStationarySites:
         """[S, 6]: amplitude, TTP, TTE, Q, Q/TTE, events."""
         out = np.zeros((self.S, 6))
         abi.check(self.lib.casim_qs_get_stats(self.h, abi.dptr(out)))
+        return out
+
+    def GetWindowStats(self):
+        """[S, 4] over [0, t_max]: amplitude, last event time, integral of N dt, N at t_max."""
+        out = np.zeros((self.S, 4))
+        abi.check(self.lib.casim_qs_get_window_stats(self.h, abi.dptr(out)))
         return out
 
     def GetFeatures(self):

@@ -42,6 +42,11 @@ def errors(net, x, y, threshold=0.02):
     if ok.any():
         out["rel_mean_unflagged"] = float(rel[ok].mean())
         out["rel_p99_unflagged"] = float(np.quantile(rel[ok], 0.99))
+    hi = true > 5.0  # mouths near open channels, where gating rates are not negligible
+    if hi.any():
+        out["rows_above_5uM"] = int(hi.sum())
+        out["rel_mean_above_5uM"] = float(rel[hi].mean())
+        out["rel_p95_above_5uM"] = float(np.quantile(rel[hi], 0.95))
     return out
 
 
@@ -50,15 +55,18 @@ def fit(x, y, width=256, depth=4, members=5, epochs=60, batch=1024, lr=1e-3, see
     net.fit_normalization(x, y)
     net.set_data(x, y)
     done = 0
-    # step decay: lr, lr/3 after schedule[0] of the epochs, lr/10 after schedule[1]
+    # step decay: lr, lr/3 after schedule[0] of the epochs, lr/10 after schedule[1];
+    # two epochs per call so that progress is reported as it goes
     for frac_end, scale in ((schedule[0], 1.0), (schedule[1], 1.0 / 3.0), (1.0, 0.1)):
-        n = max(1, int(round(epochs * frac_end)) - done)
-        t0 = time.time()
-        loss = net.train(n, batch=batch, lr=lr * scale, seed=seed + 1000 * done)
-        done += n
-        if log:
-            log({"stage": "train", "epochs": done, "lr": lr * scale, "loss": [float(v) for v in loss],
-                 "s": round(time.time() - t0, 2)})
+        end = max(done + 1, int(round(epochs * frac_end)))
+        while done < end:
+            n = min(2, end - done)
+            t0 = time.time()
+            loss = net.train(n, batch=batch, lr=lr * scale, seed=seed + 1000 * done)
+            done += n
+            if log:
+                log({"stage": "train", "epochs": done, "lr": lr * scale, "loss": [float(v) for v in loss],
+                     "s": round(time.time() - t0, 2)})
     return net
 
 
@@ -97,10 +105,15 @@ def main(argv=None):
         x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log)
     xh, yh, _ = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, tuning=tuning)
     net = fit(x, y, a.width, a.depth, a.members, a.epochs, a.batch, a.lr, log=log)
-    log({"stage": "heldout", **errors(net, xh, yh)})
-    log({"stage": "train_fit", **errors(net, x[:200_000], y[:200_000])})
+    held = errors(net, xh, yh)
+    fit_err = errors(net, x[:200_000], y[:200_000])
+    log({"stage": "heldout", **held})
+    log({"stage": "train_fit", **fit_err})
     if a.out:
-        net.save(a.out)
+        net.save(a.out, {"target": "ln mouth [Ca2+] after %d clamped steps of 10 ns from rest" % surrogate.QS_STEPS,
+                         "training": "%d batches x %d sites, %d rows; %d epochs, batch %d, Adam lr %g step-decayed"
+                                     % (a.batches, a.sites, len(y), a.epochs, a.batch, a.lr),
+                         "heldout": held, "train_fit": fit_err})
         log({"stage": "saved", "path": a.out})
     return 0
 

@@ -75,7 +75,8 @@ int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, fl
  * per event: the waiting time -ln(u)/total rate (ParallelRng::rexp), the
  * channel (cumulative rates, :113-122) and MakeTransition's draw.  The mouth
  * [Ca2+] of every RyR for the current open set is the ensemble mean of the
- * surrogate; a site whose ensemble spread exceeds spread_threshold at any RyR
+ * surrogate.  A site whose rate-weighted ensemble spread, sum over its RyRs of
+ * (rate_i / total rate) x (relative spread of Ca_i), exceeds spread_threshold
  * waits for the caller to supply the full solver's values
  * (casim_qs_set_mouths) before its next event.
  */
@@ -118,6 +119,10 @@ int casim_qs_set_mouths(CasimQS *q, int32_t site, const float *ca);
 /* Per site: amplitude, TTP, TTE, Q = integral of N dt, Q/TTE, events
  * (IndChannelCREAnalyzer.cpp:89-137 columns from the event sequence). */
 int casim_qs_get_stats(CasimQS *q, double *out);
+/* Per site, the record cut at t_max: amplitude, time of the last event,
+ * integral of N dt and N, all over [0, t_max] [S x 4]; what a fixed-step run
+ * of length t_max reports (casim_dyad_get_release_stats). */
+int casim_qs_get_window_stats(CasimQS *q, double *out);
 /* Feature rows of every channel for the current open sets [channels x 56]. */
 int casim_qs_get_features(CasimQS *q, float *out);
 /* Last ensemble mean and relative spread of every channel. */

@@ -97,3 +97,23 @@ def analyze(rec):
         q += no * (rec[nxt][0] - t)
     tte = rec[-1][0]
     return np.array([amp, ttp, tte, q, q / tte if tte > 0 else 0.0, len(rec) - 1])
+
+
+def analyze_window(rec, t_max):
+    """The record cut at t_max: amplitude, time of the last event, integral of
+    N dt and N, over [0, t_max] -- what a fixed-step run of length t_max
+    reports for the same history."""
+    amp = 0.0
+    tte = 0.0
+    q = 0.0
+    n_end = 0.0
+    for j, (t, st) in enumerate(rec):
+        if t > t_max:
+            break
+        no = float(st.sum())
+        amp = max(amp, no)
+        tte = t
+        n_end = no
+        t_next = rec[j + 1][0] if j + 1 < len(rec) else t_max
+        q += no * (min(t_next, t_max) - t)
+    return np.array([amp, tte, q, n_end])

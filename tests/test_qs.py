@@ -74,6 +74,16 @@ def test_analyze_hand_record():
     assert list(a) == [2.0, 1.0, 4.0, 6.0, 1.5, 3.0]
 
 
+def test_window_analysis_hand_record():
+    rec = [(0.0, np.array([1, 0])), (1.0, np.array([1, 1])), (3.0, np.array([0, 1])), (4.0, np.array([0, 0]))]
+    assert list(Q.analyze_window(rec, 10.0)) == [2.0, 4.0, 6.0, 0.0]
+    assert list(Q.analyze_window(rec, 2.0)) == [2.0, 1.0, 3.0, 2.0]  # 1*1 + 2*1
+    assert list(Q.analyze_window(rec, 0.5)) == [1.0, 0.0, 0.5, 1.0]
+    # open at the end of the record (the last event crossed t_max): N holds to t_max
+    rec2 = [(0.0, np.array([1])), (0.4, np.array([0])), (0.9, np.array([1])), (2.5, np.array([0]))]
+    assert list(Q.analyze_window(rec2, 2.0)) == [1.0, 0.9, 0.4 + 1.1, 1.0]
+
+
 def test_waiting_times_are_exponential():
     # constant total rate 5 channels x 0.2 / ms: mean waiting time 1 ms
     seed = O.stream_states([777] * 6, 1)[0]
@@ -143,6 +153,7 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
     net = _net(zero_bias=0.6)
     qs = surrogate.QuasiStationarySites(spec, jsr, 2.0, net, spread_threshold=0.02)
     stats = qs.RunSimulation(fallback=False)
+    win = qs.GetWindowStats()
     ca = qs.GetMouths()[0]
     states = qs.GetChannelsStates()
     par = _par(spec)
@@ -156,6 +167,9 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
         assert np.array_equal(states[s, :n], rec[-1][1]), s
         np.testing.assert_array_equal(stats[s, [0, 5]], want[[0, 5]])
         np.testing.assert_allclose(stats[s, 1:5], want[1:5], rtol=1e-12, atol=1e-15)
+        ww = Q.analyze_window(rec, 2.0)
+        np.testing.assert_array_equal(win[s, [0, 3]], ww[[0, 3]])
+        np.testing.assert_allclose(win[s, 1:3], ww[1:3], rtol=1e-12, atol=1e-15)
     assert stats[:, 5].sum() > 5 * spec.n_sites  # the runs are not trivial
     qs.close()
 
