@@ -29,10 +29,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False, tuning=None):
+def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False, tuning=None,
+                steps_per_pass=2):
     from casim_amd import autotune
     sp = workloads.batched_dyads(n_sites=n_sites, precision=precision)
     dev = DyadHIP(sp)
+    dev.SetStepsPerPass(steps_per_pass)
     dev.InitOpeningAll()
     ri, rb = sp.rim_arrays()
     dev.Update(ri, rb, sp.jsr_array([sp.meta["jsr_uM"]]))
@@ -158,6 +160,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--tuning", default=None, help="planes,threads of the state sweep (skips the autotuner)")
+    ap.add_argument("--steps-per-pass", type=int, default=2, choices=[1, 2],
+                    help="2: pairs of steps per state pass where possible (default); 1: one pass per step")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--qs-tmax", type=float, default=20.0, help="simulated ms of the surrogate leg")
     a = ap.parse_args()
@@ -170,13 +174,14 @@ def main():
     abi.load()
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
     tuning = [int(v) for v in a.tuning.split(",")] if a.tuning else None
-    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning)
+    r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning,
+                      a.steps_per_pass)
     log("fp64:", json.dumps(r64))
     r32 = None
     if not a.no_fp32:
         log("fp32 leg")
         r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune,
-                          tuning)
+                          tuning, a.steps_per_pass)
         log("fp32:", json.dumps(r32))
     sur = None
     if not a.no_surrogate:

@@ -4,7 +4,8 @@ For one batch shape it times every candidate (kernel form, threads per
 workgroup, x-planes per workgroup) with the profiled step entry point, which
 brackets each sweep launch with HIP events, and keeps the fastest.  Results
 are cached in tuning.json next to this file, keyed by grid, species layout,
-precision and site count; results of the step do not depend on the choice.
+precision, site count and steps per pass; results of the step do not depend
+on the choice.
 """
 import json
 import os
@@ -13,10 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 
 
-def key(spec):
-    return "dim%d_%dx%dx%d_i%d_b%d_f%d_p%d_S%d" % (
+def key(spec, steps_per_pass=2):
+    return "dim%d_%dx%dx%d_i%d_b%d_f%d_p%d_S%d_k%d" % (
         spec.dim, spec.nx, spec.ny, spec.nz, spec.n_ions, spec.n_bound,
-        sum(int(b.faces_only) for b in spec.buffers), spec.precision, spec.n_sites)
+        sum(int(b.faces_only) for b in spec.buffers), spec.precision, spec.n_sites, steps_per_pass)
 
 
 def candidates(spec):
@@ -45,13 +46,13 @@ def load_cache():
     return {}
 
 
-def lookup(spec):
-    return load_cache().get(key(spec))
+def lookup(spec, steps_per_pass=2):
+    return load_cache().get(key(spec, steps_per_pass))
 
 
 def apply(dev, spec):
     """Apply the cached tuning of this shape to a DyadHIP handle, if any."""
-    t = lookup(spec)
+    t = lookup(spec, dev.StepsPerPass())
     if t:
         dev.SetTuning(t["planes"], t["threads"], t.get("prefetch", 1))
     return t
@@ -74,7 +75,7 @@ def tune(dev, spec, dt, steps=20, warmup=4, save=True, log=None):
     dev.SetTuning(best["planes"], best["threads"], best["prefetch"])
     if save:
         c = load_cache()
-        c[key(spec)] = dict(best, all=res)
+        c[key(spec, dev.StepsPerPass())] = dict(best, all=res)
         with open(CACHE, "w") as f:
             json.dump(c, f, indent=1, sort_keys=True)
     return best, res
