@@ -132,6 +132,7 @@ def surrogate_leg(workloads, n_sites, t_max, plain_value):
     value = n_sites * t_max / wall
     return {"value": value, "unit": "dyad-ms/s", "speedup_vs_plain": value / plain_value, "t_max_ms": t_max,
             "wall_s": wall, "events_per_site": float(full[:, 5].mean()), "fallback_sites": int(fb),
+            "rounds": spark_compare.run_qs.last["rounds"], "fallback_s": spark_compare.run_qs.last["fallback_s"],
             "mouth_ca_rel_err_heldout": {k: held.get(k) for k in ("rel_mean", "rel_p50", "rel_p95", "rel_mean_above_5uM")},
             "target": meta.get("target")}
 

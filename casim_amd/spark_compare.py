@@ -72,9 +72,11 @@ def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None):
     wall = time.perf_counter() - t0
     win, full = qs.GetWindowStats(), qs.GetStats()
     fb = qs.fallback_solves
+    info = {"rounds": qs.rounds, "fallback_s": qs.fallback_s}
     qs.close()
     if log:
-        log({"stage": "qs", "s": round(wall, 2), "events": float(full[:, 5].sum()), "fallback_sites": fb})
+        log({"stage": "qs", "s": round(wall, 2), "events": float(full[:, 5].sum()), "fallback_sites": fb, **info})
+    run_qs.last = info
     return win, full, wall, fb
 
 

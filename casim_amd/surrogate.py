@@ -110,9 +110,21 @@ def clamped_spec(spec):
     return dataclasses.replace(spec, channel_types=types)
 
 
+def small_batch_tuning(spec, workgroups=1024):
+    """Plane chunks small enough that a batch of few sites still spreads over
+    about `workgroups` workgroups (a whole-site workgroup marches 79 planes
+    one after another)."""
+    inner = spec.nx - 2
+    chunks = min(inner, -(-workgroups // max(spec.n_sites, 1)))
+    if chunks <= 1:
+        return None
+    return (-(-inner // chunks), 0, 1)
+
+
 def qs_solve(DyadHIP, spec, states, jsr_site, steps=QS_STEPS, tuning=None):
     """Mouth [Ca2+] of every RyR after `steps` clamped steps from rest: [S, M]."""
     dev = DyadHIP(clamped_spec(spec))
+    tuning = tuning or small_batch_tuning(spec)
     if tuning:
         dev.SetTuning(*tuning)
     dev.SetChannelsStates(states)
@@ -196,6 +208,8 @@ class QuasiStationarySites:
         abi.check(self.lib.casim_qs_create(C.byref(cfg), C.byref(h)))
         self.h = h
         self.fallback_solves = 0
+        self.fallback_s = 0.0
+        self.rounds = 0
 
     def __del__(self):
         self.close()
@@ -208,6 +222,8 @@ class QuasiStationarySites:
     def Reset(self):
         abi.check(self.lib.casim_qs_reset(self.h))
         self.fallback_solves = 0
+        self.fallback_s = 0.0
+        self.rounds = 0
 
     def Round(self, max_rounds=1, wait_batch=1 << 30):
         """Up to max_rounds events per active, non-waiting site;
@@ -277,16 +293,20 @@ class QuasiStationarySites:
     def RunSimulation(self, max_rounds=1 << 30, fallback=True, wait_batch=256, tuning=None):
         """Run every site to its end; returns GetStats().  Sites that ask for
         the full solver are resolved in batches of up to wait_batch."""
+        import time
         done = 0
         while done < max_rounds:
             r, a, w = self.Round(max_rounds - done, wait_batch)
             done += r
+            self.rounds += r
             if a == 0:
                 break
             if w:
                 if not fallback:
                     raise RuntimeError("%d sites need the full solver and fallback is off" % w)
+                t0 = time.perf_counter()
                 self.resolve_waiting(tuning)
+                self.fallback_s += time.perf_counter() - t0
             elif r == 0:
                 break
         return self.GetStats()
