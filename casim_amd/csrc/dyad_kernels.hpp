@@ -363,7 +363,8 @@ __device__ __forceinline__ void lds_barrier() {
 // Species layouts the sweeps can take at compile time: bound field b binds
 // ion bion[b], belongs to buffer bbuf[b], lives on the z-faces only when
 // bface[b] and sits in grid (or face) slot bslot[b]; ufaces[u]: buffer u
-// exists on the two z-faces only.  LY = 0 reads them from Params at run time.
+// exists on the two z-faces only; the fixed layouts have no sarcolemmal
+// buffer and diffusing A and B.  LY = 0 reads all of it from Params.
 //  1: one ion; buffers A, B mobile, F on the z-faces (Ca-A, Ca-B, Ca-F)
 //  2: one ion; buffers A, B mobile, F fixed on every node
 template <int LY>
@@ -397,6 +398,14 @@ template <int LY> __device__ __forceinline__ int ly_bface(const Params &p, int b
 }
 template <int LY> __device__ __forceinline__ int ly_bslot(const Params &p, int b) {
     if constexpr (Layout<LY>::fixed) return Layout<LY>::bslot[b]; else return p.bslot[b];
+}
+// sarcolemmal buffer of ion j present (Params::sl)
+template <int LY> __device__ __forceinline__ bool ly_sl(const Params &p, int j) {
+    if constexpr (Layout<LY>::fixed) return false; else return p.sl[j] != 0;
+}
+// bound field b diffuses (a grid field with D != 0)
+template <int LY> __device__ __forceinline__ bool ly_mobile(const Params &p, int b) {
+    if constexpr (Layout<LY>::fixed) return !Layout<LY>::bface[b] && b < 2; else return p.Db[b] != 0.0;
 }
 template <int LY> __device__ __forceinline__ double ly_total(const Params &p, int u, int z) {
     int faces;
@@ -451,7 +460,7 @@ __device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], 
                     cxl = Cm[gg];
                 }
             Bv[b] = vc;
-            if (p.Db[b] != 0.0) {
+            if (ly_mobile<LY>(p, b)) {
                 const T *u = L + g * P + c;
                 T lap = (cxr - 2 * vc + cxl) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
                 if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
@@ -494,7 +503,7 @@ __device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], 
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
         T e = eC[j] + J[j];
-        if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
+        if ((DIM == 2 || z == p.nz - 1) && ly_sl<LY>(p, j)) {
             T s1 = (T)p.K1[j] + Cv[j];
             T s2 = (T)p.K2[j] + Cv[j];
             e *= (T)1 / ((T)1 + (T)p.N1[j] * (T)p.K1[j] / (s1 * s1) + (T)p.N2[j] * (T)p.K2[j] / (s2 * s2));
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
     const int x0 = 1 + cidx * chunk;
     const int x1 = min(p.nx - 1, x0 + chunk);
     const int P = p.ny * p.nz;
-    const int ng = p.n_grid, nfc = p.n_face;
+    const int ng = NGT ? NGT : p.n_grid, nfc = NGT ? NFT : p.n_face;  // compile-time with the layout
     const size_t fs = (size_t)p.S * p.NS;
     const T *gs = gin + (size_t)s * p.NS;
     T *os = gout + (size_t)s * p.NS;
@@ -726,7 +735,7 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
     const int x1 = min(p.nx - 1, x0 + chunk);  // step k+2 planes [x0, x1)
     const int xa0 = x0 - 1;                    // step k+1 planes [xa0, x1]
     const int P = p.ny * p.nz;
-    const int ng = p.n_grid, nfc = p.n_face;
+    const int ng = NGT ? NGT : p.n_grid, nfc = NGT ? NFT : p.n_face;  // compile-time with the layout
     const size_t fs = (size_t)p.S * p.NS;
     const T *gs = gin + (size_t)s * p.NS;
     T *os = gout + (size_t)s * p.NS;

@@ -39,7 +39,8 @@ void sweep_slots(CasimDyad *d, double dt) {
 
 // The compile-time species layout (Layout<LY>) matching the run-time one, or 0.
 int layout_id(const Params &p) {
-    if (p.ni != 1 || p.nb != 3 || p.nu != 3) return 0;
+    if (p.ni != 1 || p.nb != 3 || p.nu != 3 || p.sl[0] || p.Db[0] == 0.0 || p.Db[1] == 0.0 || p.Db[2] != 0.0)
+        return 0;
     for (int ly = 1; ly <= 2; ++ly) {
         bool ok = true;
         for (int b = 0; b < 3; ++b) {
