@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "simulated dyad-ms/s on 1 GPU (plain vs surrogate) at ≤2% Ca err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+COPY_GBS = 5140.0  # a plain 1 GiB device copy on this MI355X (casim_amd/copy_probe.py, rocprofv3 trace)
 
 
 def log(*a):
@@ -198,6 +199,7 @@ def main():
     kernel = "k_sweep2" if r64["steps_per_launch"] == 2 else "k_sweep"
     launch_bytes = r64["steps_per_launch"] * a.sites * r64["bytes_per_site_step"]
     achieved = launch_bytes / (r64["sweep_launch_ms"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(kernel)
     out = {
         "metric": METRIC,
         "value": r64["dyad_ms_per_s"],
@@ -215,9 +217,12 @@ def main():
                    "path": "plain batched RD, graph of %d steps" % min(a.chunk, a.steps),
                    "tuning": r64["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(kernel),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "steps_per_launch": r64["steps_per_launch"],
-                     "launch_ms": r64["sweep_launch_ms"]},
+                     "launch_ms": r64["sweep_launch_ms"],
+                     # what HBM actually moves per launch (traffic / launch time), next to the copy rate
+                     "physical_gbs": None if traffic is None else traffic / (r64["sweep_launch_ms"] * 1e-3) / 1e9,
+                     "copy_gbs": COPY_GBS},
         "cpu_baseline": cpu,
         "hbm_gbs_whole_step": a.sites * r64["bytes_per_site_step"] / (r64["ms_per_step"] * 1e-3) / 1e9,
         "fp32": None if r32 is None else {
