@@ -87,3 +87,26 @@ class StatsTracker:
     def table(self):
         qm = np.where(self.tte > 0, self.q / np.where(self.tte > 0, self.tte, 1.0), 0.0)
         return np.stack([self.amp, self.ttp, self.tte, self.q, qm, self.n], axis=1)
+
+
+def time_histograms(counts, amplitude, n_channels, nadir=4):
+    """IndChannelCREAnalyzer's time histograms (IndChannelCREAnalyzer.cpp:53-137,
+    CalculateGlobalHist :24-45) from per-bin open counts [bins, S] (255 = bin
+    never reached, counted as nothing open): for every bin, the number of
+    sites with k channels open, k >= 1, and in column 0 the remaining sites;
+    for all sites and separately for quarks (amplitude 1), blips (amplitude
+    <= nadir) and sparks."""
+    counts = np.asarray(counts)
+    amp = np.asarray(amplitude)
+    classes = {"all": np.ones_like(amp, dtype=bool), "quarks": amp == 1,
+               "blips": (amp != 1) & (amp <= nadir), "sparks": amp > nadir}
+    width = {"all": n_channels + 1, "quarks": 2, "blips": nadir + 1, "sparks": n_channels + 1}
+    out = {}
+    for name, mask in classes.items():
+        c = counts[:, mask].astype(np.int64)
+        h = np.zeros((counts.shape[0], width[name]), dtype=np.int64)
+        for k in range(1, width[name]):
+            h[:, k] = (c == k).sum(axis=1)
+        h[:, 0] = int(mask.sum()) - h[:, 1:].sum(axis=1)
+        out[name] = h
+    return out

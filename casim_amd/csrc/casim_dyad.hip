@@ -462,6 +462,9 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         Ch.sJ = d->d_sJ;
         Ch.totals = d->d_totals;
         Ch.stats = d->d_stats;
+        Ch.trace = nullptr;
+        Ch.trace_every = 1;
+        Ch.trace_bins = 0;
         HIPCK(hipStreamSynchronize(d->stream));
         if (casim_dyad_reset(d) != CASIM_OK) throw Err(CASIM_E_DEVICE, casim::last_error());
         *out = d;
@@ -481,7 +484,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     d->clear_graphs();
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
                     d->d_sperm, d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
-                    d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats};
+                    d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats, d->Ch.trace};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : d->pool_buffers)
@@ -507,6 +510,8 @@ int casim_dyad_reset(CasimDyad *d) {
         HIPCK(hipGetLastError());
         HIPCK(hipMemsetAsync(d->d_state, 0, sizeof(int) * std::max(1, d->nch), d->stream));
         HIPCK(hipMemsetAsync(d->d_stats, 0, sizeof(double) * NSTAT * P.S, d->stream));
+        if (d->Ch.trace)
+            HIPCK(hipMemsetAsync(d->Ch.trace, 0xFF, (size_t)d->Ch.trace_bins * P.S, d->stream));
         if (d->Q.on) {
             hipLaunchKernelGGL(k_pools_reset, dim3((P.S + 127) / 128), dim3(128), 0, d->stream, d->Q, P.S, P.ni,
                                d->d_jc0);
@@ -1116,6 +1121,36 @@ int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t thread
             d->use_sweep = true;
         }
         d->clear_graphs();
+    });
+}
+
+int casim_dyad_record_open_counts(CasimDyad *d, int32_t steps_per_bin, int32_t max_bins) {
+    return guard([&] {
+        REQUIRE(d, CASIM_E_INVALID, "null handle");
+        REQUIRE(steps_per_bin >= 1 && max_bins >= 0, CASIM_E_INVALID, "steps_per_bin >= 1, max_bins >= 0");
+        d->use();
+        HIPCK(hipStreamSynchronize(d->stream));
+        if (d->Ch.trace) (void)hipFree(d->Ch.trace);
+        d->Ch.trace = nullptr;
+        d->Ch.trace_every = steps_per_bin;
+        d->Ch.trace_bins = max_bins;
+        if (max_bins > 0) {
+            HIPCK(hipMalloc(&d->Ch.trace, (size_t)max_bins * d->P.S));
+            HIPCK(hipMemsetAsync(d->Ch.trace, 0xFF, (size_t)max_bins * d->P.S, d->stream));
+            HIPCK(hipStreamSynchronize(d->stream));
+        }
+        d->clear_graphs();
+    });
+}
+
+int casim_dyad_get_open_counts(CasimDyad *d, uint8_t *out) {
+    return guard([&] {
+        REQUIRE(d && out, CASIM_E_INVALID, "null argument");
+        REQUIRE(d->Ch.trace, CASIM_E_STATE, "open-count recording is off");
+        d->use();
+        HIPCK(hipMemcpyAsync(out, d->Ch.trace, (size_t)d->Ch.trace_bins * d->P.S, hipMemcpyDeviceToHost,
+                             d->stream));
+        HIPCK(hipStreamSynchronize(d->stream));
     });
 }
 

@@ -57,6 +57,8 @@ struct Channels {
     double *sJ;         // [ni*nch] currents in node-sorted order
     double *totals;     // [ni*S]
     double *stats;      // [S*NSTAT] release statistics, see StatSlot
+    uint8_t *trace;     // [bins][S] open channels sampled every trace_every steps (or null)
+    int trace_every, trace_bins;
     int M;              // max channels per site
     int nch;            // channels over all sites
     int ca, mg;
@@ -1156,6 +1158,13 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
         st[ST_Q] += nn * dt;
         st[ST_NOPEN] = nn;
         st[ST_STEPS] = steps;
+        // time histogram input (IndChannelCREAnalyzer.cpp:69-84 on per-step records):
+        // bin k >= 1 holds N after step k*every - 1, bin 0 the state after InitOpening
+        if (a.trace) {
+            const long long m1 = (long long)steps + 1;
+            if (m1 % a.trace_every == 0 && m1 / a.trace_every < a.trace_bins)
+                a.trace[(size_t)(m1 / a.trace_every) * p.S + s] = (uint8_t)(nopen < 254 ? nopen : 254);
+        }
         const uint32_t *J = a.jump + (size_t)pos * 18;
         uint32_t out[6];
         mrg::matvec(J, S6, out, mrg::C1, mrg::M1);
@@ -1178,6 +1187,7 @@ static __global__ void k_init_opening(Channels a, int S, int site, int channel, 
     for (int i = 0; i < NSTAT; ++i) st[i] = 0.0;
     st[ST_NOPEN] = n0;
     st[ST_AMP] = n0;
+    if (a.trace) a.trace[s] = (uint8_t)n0;
     uint32_t S6[6], out[6];
     for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
     const uint32_t *J = a.jump + (size_t)(n + 1) * 18;

@@ -36,6 +36,10 @@ struct QsDev {
     int *active;   // 1 while the site evolves
     int *waiting;  // 0 no, 1 needs the full solver, 2 values supplied
     int *crow;     // [S] first surrogate row of the site this round, -1 if none
+    uint8_t *trace;  // [bins][S] open channels per time bin (or null)
+    int *cur_bin;    // [S] first bin not yet written
+    double bin_ms;
+    int bins;
     int R;         // surrogate rows this round (member stride of out)
     double *stats;
     const double *jsr;
@@ -195,6 +199,15 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
         pc += w;
     }
     (void)mrg::draw(S6);  // MakeTransition(runif())
+    if (q.trace) {
+        // IndChannelCREAnalyzer.cpp:78-84: bins up to the next record's time
+        // take the current count
+        const long long nxt = (long long)(t / q.bin_ms);
+        const long long end = nxt < q.bins - 1 ? nxt : q.bins - 1;
+        const uint8_t v = (uint8_t)(st[QS_NCUR] < 254.0 ? st[QS_NCUR] : 254.0);
+        for (long long k = q.cur_bin[s]; k <= end; ++k) q.trace[(size_t)k * q.S + s] = v;
+        q.cur_bin[s] = (int)(nxt + 1 < (long long)q.bins ? nxt + 1 : q.bins);
+    }
     const int was = q.state[off + idx];
     q.state[off + idx] = was ^ 1;
     const double ncur = st[QS_NCUR];
@@ -234,6 +247,7 @@ __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t 
     mrg::matvec(J + 9, S6 + 3, o6 + 3, mrg::C2, mrg::M2);
     for (int k = 0; k < 6; ++k) q.rng[s * 6 + k] = o6[k];
     q.t[s] = 0.0;
+    q.cur_bin[s] = 0;
     q.active[s] = q.t_max > 0.0;
     q.waiting[s] = 0;
     double *st = q.stats + (size_t)s * QS_NSTAT;
@@ -396,6 +410,10 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.thresh = (float)c->spread_threshold;
         h->d_init = h->alloc<int>(S);
         h->d_count = h->alloc<int>(4);
+        q.cur_bin = h->alloc<int>(S);
+        q.trace = nullptr;
+        q.bins = 0;
+        q.bin_ms = 0.01;
         q.crow = h->alloc<int>(S);
         h->d_rng0 = h->alloc<uint32_t>((size_t)S * 6);
         h->d_jump = h->alloc<uint32_t>(jump.size());
@@ -423,6 +441,7 @@ int casim_qs_destroy(CasimQS *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->bufs) (void)hipFree(p);
     if (h->q.out) (void)hipFree(h->q.out);
+    if (h->q.trace) (void)hipFree(h->q.trace);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CASIM_OK;
@@ -432,6 +451,7 @@ int casim_qs_reset(CasimQS *h) {
     return guard([&] {
         REQUIRE(h, CASIM_E_INVALID, "null handle");
         h->use();
+        if (h->q.trace) HIPCK(hipMemsetAsync(h->q.trace, 0xFF, (size_t)h->q.bins * h->q.S, h->stream));
         hipLaunchKernelGGL(k_qs_init, dim3((h->q.S + 127) / 128), dim3(128), 0, h->stream, h->q, h->d_init, h->d_rng0,
                            h->d_jump);
         HIPCK(hipGetLastError());
@@ -533,6 +553,34 @@ int casim_qs_get_stats(CasimQS *h, double *out) {
             o[4] = a[QS_TTE] > 0 ? a[QS_Q] / a[QS_TTE] : 0.0;
             o[5] = a[QS_EVENTS];
         }
+    });
+}
+
+int casim_qs_record_open_counts(CasimQS *h, double bin_ms, int32_t max_bins) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        REQUIRE(bin_ms > 0 && max_bins >= 0, CASIM_E_INVALID, "bin_ms > 0, max_bins >= 0");
+        h->use();
+        HIPCK(hipStreamSynchronize(h->stream));
+        if (h->q.trace) (void)hipFree(h->q.trace);
+        h->q.trace = nullptr;
+        h->q.bins = max_bins;
+        h->q.bin_ms = bin_ms;
+        if (max_bins > 0) {
+            HIPCK(hipMalloc(&h->q.trace, (size_t)max_bins * h->q.S));
+            HIPCK(hipMemsetAsync(h->q.trace, 0xFF, (size_t)max_bins * h->q.S, h->stream));
+            HIPCK(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+int casim_qs_get_open_counts(CasimQS *h, uint8_t *out) {
+    return guard([&] {
+        REQUIRE(h && out, CASIM_E_INVALID, "null argument");
+        REQUIRE(h->q.trace, CASIM_E_STATE, "open-count recording is off");
+        h->use();
+        HIPCK(hipMemcpyAsync(out, h->q.trace, (size_t)h->q.bins * h->q.S, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
     });
 }
 

@@ -174,6 +174,18 @@ class DyadHIP:
     def SetTuning(self, planes=0, threads=0, prefetch=0):
         abi.check(self.lib.casim_dyad_set_tuning(self.h, planes, threads, prefetch))
 
+    def RecordOpenCounts(self, steps_per_bin, max_bins):
+        """Open-channel counts per site every steps_per_bin steps (bin 0:
+        after InitOpening), for the analyzer's time histograms."""
+        abi.check(self.lib.casim_dyad_record_open_counts(self.h, steps_per_bin, max_bins))
+        self._bins = max_bins
+
+    def GetOpenCounts(self):
+        """[bins, S] uint8, 255 where a bin was not reached."""
+        out = np.empty((self._bins, self.spec.n_sites), dtype=np.uint8)
+        abi.check(self.lib.casim_dyad_get_open_counts(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def SetStepsPerPass(self, steps):
         abi.check(self.lib.casim_dyad_set_steps_per_pass(self.h, steps))
 

@@ -32,12 +32,17 @@ def workload(n_sites, geometry_seed=2024, kinetics="reference", precision=abi.CA
     return spec
 
 
+BIN_MS = 0.01  # IndChannelCREAnalyzer's time-histogram bin (IndChannelCREAnalyzer.cpp:6-8)
+
+
 def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None):
-    """Full RD + gating for t_max ms: release stats [S, 6], wall seconds."""
+    """Full RD + gating for t_max ms: release stats [S, 6], wall seconds;
+    per-bin open counts in run_plain.counts."""
     from .dyad import DyadHIP
     dt = workloads.DT_MS
     steps = int(round(t_max / dt))
     dev = DyadHIP(spec)
+    dev.RecordOpenCounts(int(round(BIN_MS / dt)), int(round(t_max / BIN_MS)) + 1)
     dev.InitOpeningAll()
     ri, rb = spec.rim_arrays()
     dev.Update(ri, rb, spec.jsr_array([jsr]))
@@ -58,6 +63,7 @@ def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None):
     dev.Synchronize()
     wall = time.perf_counter() - t0
     stats = dev.GetReleaseStats()
+    run_plain.counts = dev.GetOpenCounts()
     dev.close()
     return stats, wall
 
@@ -67,12 +73,14 @@ def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None):
     event-record stats [S, 6], wall seconds, fallback solves."""
     from .surrogate import QuasiStationarySites
     qs = QuasiStationarySites(spec, np.full(spec.n_sites, jsr), t_max, net, spread_threshold=threshold)
+    qs.RecordOpenCounts(BIN_MS, int(round(t_max / BIN_MS)) + 1)
     t0 = time.perf_counter()
     qs.RunSimulation()
     wall = time.perf_counter() - t0
     win, full = qs.GetWindowStats(), qs.GetStats()
     fb = qs.fallback_solves
     info = {"rounds": qs.rounds, "fallback_s": qs.fallback_s}
+    run_qs.counts = qs.GetOpenCounts()
     qs.close()
     if log:
         log({"stage": "qs", "s": round(wall, 2), "events": float(full[:, 5].sum()), "fallback_sites": fb, **info})
@@ -110,6 +118,22 @@ def compare(plain, qs_win, n_channels, t_max, alpha=0.01, dt=workloads.DT_MS):
     return out
 
 
+def compare_histograms(counts_plain, counts_qs, amp_plain, amp_qs, n_channels, nadir=4):
+    """Mean open count against time (the analyzer's time histograms reduced
+    to their first moment), all sites and per class."""
+    out = {}
+    hp = analysis.time_histograms(counts_plain, amp_plain, n_channels, nadir)
+    hq = analysis.time_histograms(counts_qs, amp_qs, n_channels, nadir)
+    for name in ("all", "quarks", "blips", "sparks"):
+        def mean_curve(h):
+            n = h.sum(axis=1)
+            return (h * np.arange(h.shape[1])).sum(axis=1) / np.maximum(n, 1)
+        a, b = mean_curve(hp[name]), mean_curve(hq[name])
+        out[name] = {"max_abs_diff": float(np.max(np.abs(a - b))), "plain_end": float(a[-1]), "qs_end": float(b[-1]),
+                     "sites_plain": int(hp[name][0].sum()), "sites_qs": int(hq[name][0].sum())}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--sites", type=int, default=1024)
@@ -135,7 +159,9 @@ def main(argv=None):
     if not a.skip_plain:
         plain, wp = run_plain(spec, a.t_max, a.jsr, log=log)
         res.update({"plain_wall_s": wp, "plain_dyad_ms_per_s": a.sites * a.t_max / wp,
-                    "speedup": wp / wq, "ks": compare(plain, win, n_ch, a.t_max)})
+                    "speedup": wp / wq, "ks": compare(plain, win, n_ch, a.t_max),
+                    "mean_open_vs_time": compare_histograms(run_plain.counts, run_qs.counts, plain[:, 0], win[:, 0],
+                                                            int(n_ch.max()))})
     log(res)
     return 0
 

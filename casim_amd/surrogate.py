@@ -252,6 +252,17 @@ class QuasiStationarySites:
         abi.check(self.lib.casim_qs_get_stats(self.h, abi.dptr(out)))
         return out
 
+    def RecordOpenCounts(self, bin_ms, max_bins):
+        """Per-bin open counts for the analyzer's time histograms; takes
+        effect from the next Reset."""
+        abi.check(self.lib.casim_qs_record_open_counts(self.h, float(bin_ms), int(max_bins)))
+        self._bins = max_bins
+
+    def GetOpenCounts(self):
+        out = np.empty((self._bins, self.S), dtype=np.uint8)
+        abi.check(self.lib.casim_qs_get_open_counts(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def GetWindowStats(self):
         """[S, 4] over [0, t_max]: amplitude, last event time, integral of N dt, N at t_max."""
         out = np.zeros((self.S, 4))

@@ -309,6 +309,16 @@ double casim_dyad_bytes_per_site_step(const CasimDyad *d);
  * the tuning. */
 int casim_dyad_set_tuning(CasimDyad *d, int32_t planes_per_block, int32_t threads_per_block, int32_t prefetch);
 
+/* Time-histogram input of IndChannelCREAnalyzer::RunAnalysis
+ * (IndChannelCREAnalyzer.cpp:53-137, bin width dt = 0.01 ms there): with
+ * steps_per_bin = B, bin 0 holds each site's open-channel count after
+ * InitOpening and bin k >= 1 the count after step k*B - 1 (what the analyzer
+ * assigns to bin k from per-step records), for k < max_bins; 255 marks bins
+ * not reached.  max_bins = 0 turns recording off.  Reset clears the bins. */
+int casim_dyad_record_open_counts(CasimDyad *d, int32_t steps_per_bin, int32_t max_bins);
+/* The recorded counts, [max_bins][S] bytes. */
+int casim_dyad_get_open_counts(CasimDyad *d, uint8_t *out);
+
 /* Steps per state pass of casim_dyad_step: 2 (the default) advances pairs of
  * steps in one read and one write of the state, where the configuration
  * allows it (plane-marching kernel, no pools attached); 1 forces one pass per

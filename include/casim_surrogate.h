@@ -123,6 +123,14 @@ int casim_qs_get_stats(CasimQS *q, double *out);
  * integral of N dt and N, all over [0, t_max] [S x 4]; what a fixed-step run
  * of length t_max reports (casim_dyad_get_release_stats). */
 int casim_qs_get_window_stats(CasimQS *q, double *out);
+/* Time-histogram input of IndChannelCREAnalyzer::RunAnalysis
+ * (IndChannelCREAnalyzer.cpp:69-117): at each event at time t, bins from the
+ * first unwritten one up to floor(t / bin_ms) (at most max_bins - 1) take the
+ * open count before the event; 255 marks bins never written.  Set before
+ * casim_qs_reset (which clears the bins); max_bins = 0 turns it off. */
+int casim_qs_record_open_counts(CasimQS *q, double bin_ms, int32_t max_bins);
+/* The recorded counts, [max_bins][S] bytes. */
+int casim_qs_get_open_counts(CasimQS *q, uint8_t *out);
 /* Feature rows of every channel for the current open sets [channels x 56]. */
 int casim_qs_get_features(CasimQS *q, float *out);
 /* Last ensemble mean and relative spread of every channel. */

@@ -117,3 +117,19 @@ def analyze_window(rec, t_max):
         t_next = rec[j + 1][0] if j + 1 < len(rec) else t_max
         q += no * (min(t_next, t_max) - t)
     return np.array([amp, tte, q, n_end])
+
+
+def open_count_bins(rec, bin_dt, n_bins):
+    """Per-bin open counts of one record as IndChannelCREAnalyzer fills its
+    time histogram (IndChannelCREAnalyzer.cpp:69-84, WriteIntoHistogram
+    :18-22): record j covers bins from the first unwritten one up to
+    trunc(t_{j+1} / dt) (the last record covers none); 255 = never written."""
+    out = np.full(n_bins, 255, dtype=np.uint8)
+    cur = 0
+    for j, (t, st) in enumerate(rec):
+        nxt = rec[min(j + 1, len(rec) - 1)][0]
+        nidx = int(nxt / bin_dt)
+        for k in range(cur, min(nidx, n_bins - 1) + 1):
+            out[k] = int(st.sum())
+        cur = nidx + 1
+    return out

@@ -268,6 +268,35 @@ def test_release_stats_match_oracle_trajectory(gpu, oracle_mod):
     assert want[:, 0].max() > 1
 
 
+def test_open_count_trace_matches_oracle(gpu, oracle_mod):
+    """Per-bin open counts (the analyzer's time-histogram input): bin 0 after
+    InitOpening, bin k the count after step k*B - 1, through paired passes."""
+    sp = workloads.single_dyad(6, clamped=False)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    sp.channel_types[0].params["kclose0"] *= 1e3
+    ora = oracle_mod.Oracle(sp)
+    dev = DyadHIP(sp)
+    B, bins = 7, 40
+    dev.RecordOpenCounts(B, bins)
+    ora.init_opening()
+    dev.InitOpeningAll()
+    ri, rb = sp.rim_arrays()
+    j = sp.jsr_array([1000.0])
+    ora.update(ri, rb, j)
+    dev.Update(ri, rb, j)
+    dt = 1e-5
+    want = np.full((bins, sp.n_sites), 255, dtype=np.uint8)
+    want[0] = (ora.channel_states() == 1).sum(axis=1)
+    for m in range(1, 200):
+        ora.step(dt, 1, threads=4)
+        if (m + 1) % B == 0 and (m + 1) // B < bins:
+            want[(m + 1) // B] = (ora.channel_states() == 1).sum(axis=1)
+    dev.Step(dt, 199)
+    got = dev.GetOpenCounts()
+    assert np.array_equal(got, want)
+    assert len(np.unique(want[want != 255])) > 2
+
+
 def _pool_pair(oracle_mod, sp, ps, rim=None):
     ora = oracle_mod.OracleHybridRun(sp, ps, rim=rim)
     ora.reset()

@@ -84,6 +84,26 @@ def test_window_analysis_hand_record():
     assert list(Q.analyze_window(rec2, 2.0)) == [1.0, 0.9, 0.4 + 1.1, 1.0]
 
 
+def test_open_count_bins_hand_record():
+    rec = [(0.0, np.array([1, 0])), (0.025, np.array([1, 1])), (0.031, np.array([0, 1])),
+           (0.035, np.array([0, 0]))]
+    # bins 0..2 hold the first record (next event at 0.025 -> bin 2), bin 3 the
+    # second (next at 0.031), the third covers no new bin, the last none
+    got = Q.open_count_bins(rec, 0.01, 6)
+    assert list(got) == [1, 1, 1, 2, 255, 255]
+
+
+def test_time_histograms_hand_counts():
+    from casim_amd.analysis import time_histograms
+    counts = np.array([[1, 1, 1], [2, 0, 1], [255, 0, 3]], dtype=np.uint8)  # [bins, sites]
+    amp = np.array([2, 1, 3])
+    h = time_histograms(counts, amp, n_channels=3, nadir=2)
+    assert h["all"].tolist() == [[0, 3, 0, 0], [1, 1, 1, 0], [2, 0, 0, 1]]
+    assert h["quarks"].tolist() == [[0, 1], [1, 0], [1, 0]]
+    assert h["blips"].tolist() == [[0, 1, 0], [0, 0, 1], [1, 0, 0]]
+    assert h["sparks"].tolist() == [[0, 1, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1]]
+
+
 def test_waiting_times_are_exponential():
     # constant total rate 5 channels x 0.2 / ms: mean waiting time 1 ms
     seed = O.stream_states([777] * 6, 1)[0]
@@ -152,8 +172,10 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
     jsr = np.full(spec.n_sites, 800.0)
     net = _net(zero_bias=0.6)
     qs = surrogate.QuasiStationarySites(spec, jsr, 2.0, net, spread_threshold=0.02)
+    qs.RecordOpenCounts(0.01, 201)
     stats = qs.RunSimulation(fallback=False)
     win = qs.GetWindowStats()
+    bins = qs.GetOpenCounts()
     ca = qs.GetMouths()[0]
     states = qs.GetChannelsStates()
     par = _par(spec)
@@ -167,6 +189,7 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
         assert np.array_equal(states[s, :n], rec[-1][1]), s
         np.testing.assert_array_equal(stats[s, [0, 5]], want[[0, 5]])
         np.testing.assert_allclose(stats[s, 1:5], want[1:5], rtol=1e-12, atol=1e-15)
+        assert np.array_equal(bins[:, s], Q.open_count_bins(rec, 0.01, 201)), s
         ww = Q.analyze_window(rec, 2.0)
         np.testing.assert_array_equal(win[s, [0, 3]], ww[[0, 3]])
         np.testing.assert_allclose(win[s, 1:3], ww[1:3], rtol=1e-12, atol=1e-15)
