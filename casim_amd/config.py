@@ -221,6 +221,15 @@ class DyadSpec:
         return out
 
 
+def element_volume(spec):
+    """Node volume the channel currents refer to (DyadRDCPU::GetElementVolume,
+    DyadRDCPU.cpp:195-197, 0.5 dx dy dz in 3D; DyadRD2DCPU.cpp:189-191,
+    dx dy z in 2D), nm^3."""
+    if spec.dim == 3:
+        return 0.5 * spec.dx * spec.dy * spec.dz
+    return spec.dx * spec.dy * spec.z
+
+
 def _sorted_items(obj):
     return sorted(obj.items(), key=lambda kv: kv[0])
 

@@ -123,7 +123,8 @@ void CasimDyad::launch_step(double dt) {
 }
 
 bool CasimDyad::pair_ok() const {
-    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || Q.on) return false;
+    // a cytosol pool moves the Dirichlet rim every step: one step per pass
+    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || (Q.on && Q.cyto)) return false;
     return 4 * (size_t)P.n_grid * P.ny * P.nz * tsize <= 64 * 1024;
 }
 
@@ -775,7 +776,8 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                         // steps k and k+1 in one pass, then step k+1's channel
                         // update from the mouth values the pass stored
                         d->launch_pair(dt);
-                        d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE, dt);
+                        d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                           dt);
                         k += 2;
                     } else {
                         d->launch_step(dt);
@@ -834,7 +836,8 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                     d->launch_pair(dt);
                     HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
                     HIPCK(hipEventRecord(ev[4 * (k + 1)], d->stream));
-                    d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE, dt);
+                    d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                           dt);
                     HIPCK(hipEventRecord(ev[4 * (k + 1) + 1], d->stream));
                     HIPCK(hipEventRecord(ev[4 * (k + 1) + 2], d->stream));
                     HIPCK(hipEventRecord(ev[4 * (k + 1) + 3], d->stream));

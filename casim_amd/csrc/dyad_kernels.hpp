@@ -709,7 +709,8 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 // channel on a node winning; the mouth values of step k+1 (ions, bound
 // fields, free buffers) are written to mouths[] for the next channel call,
 // which gates step k+2 from them.  Rim nodes stay at their step-k values
-// (constant Dirichlet rim).  Chunks of one site recompute one halo plane of
+// (constant Dirichlet rim: no cytosol pool); with a jSR pool the currents use
+// the pool's value, which the channel call before the pass has advanced.  Chunks of one site recompute one halo plane of
 // step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
 // two steps here equal two k_sweep steps bit for bit.
 template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL>
@@ -718,7 +719,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                                                 const int *__restrict__ ch_off, const int *__restrict__ snode,
                                                 const int *__restrict__ sperm, const double *__restrict__ sJ,
                                                 const int *__restrict__ cstate, const int *__restrict__ ctype,
-                                                const double *__restrict__ jsr, double *__restrict__ mouths,
+                                                const double *__restrict__ jsr, const double *__restrict__ jnew,
+                                                double *__restrict__ mouths,
                                                 const Channels a, Params p, double dt_, int chunk, int nchunks,
                                                 int units) {
     constexpr int LI = NI ? NI : MAXI;
@@ -893,7 +895,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                             if (!NI && j >= p.ni) break;
                             const double ion = (double)A[1][k][j];
                             mo[(size_t)a.M * s + j * MS] = ion;
-                            const double jv = jsr[i + (size_t)a.M * (s + (size_t)j * p.S)];
+                            // jSR of the step: the device pool's value when pools are on
+                            const double jv = jnew ? jnew[s + j * p.S] : jsr[i + (size_t)a.M * (s + (size_t)j * p.S)];
                             const double cond = a.g[typ * MAXI + j] * (jv - ion);
                             J[k][j] = (T)((double)st * cond);
                         }

@@ -42,7 +42,10 @@ struct QsDev {
     int bins;
     int R;         // surrogate rows this round (member stride of out)
     double *stats;
-    const double *jsr;
+    double *jsr;      // [S] current jSR
+    double *jsr_min;  // [S] lowest jSR reached
+    int deplete;
+    double Vr, cq, ncq, kcq, tau, nsr, gch;
     float *X;        // [channels x QS_F]
     float *out;      // [members x channels]
     float *ca;       // [channels] mouth [Ca2+] in use
@@ -124,13 +127,31 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
     }
 }
 
+// SRPool::Run (SRPool.cpp:70-89) over an interval dt with a fixed open set:
+// I(c) = c * geff (mouth [Ca2+] proportional to c within the interval),
+// explicit steps of at most 1 us, operations in the pool's order.
+__device__ double jsr_advance(const QsDev &q, double c, double geff, double dt) {
+#pragma clang fp contract(off)
+    double n = ceil(dt / 1e-3);
+    if (n < 1.0) n = 1.0;
+    if (n > 1e6) n = 1e6;
+    const double h = dt / n;
+    for (double k = 0; k < n; k += 1.0) {
+        const double kc = q.kcq + c;
+        const double beta = 1 + q.ncq * q.kcq * q.cq / (kc * kc);
+        const double Iref = (q.nsr - c) / q.tau;
+        c = c + h / beta * (Iref - c * geff * q.Vr);
+    }
+    return c;
+}
+
 // Ensemble mean / spread of every channel of active sites, then one Gillespie
 // event per site unless the surrogate is too uncertain for it: the site asks
 // for the full solver when the rate-weighted mean of the relative spreads,
 // sum_i (rate_i / total) spread_i, exceeds the threshold.
 __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     __shared__ double rate[QS_MAXCH];
-    __shared__ float spv[QS_MAXCH];
+    __shared__ float spv[QS_MAXCH], cav[QS_MAXCH];
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
     const int base = q.crow[s];
@@ -157,6 +178,7 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
         q.ca[row] = ca;
         q.spread[row] = sp;
         spv[i] = sp;
+        cav[i] = ca;
         const int st = q.state[row];
         rate[i] = channel_rate(q.model, q.par, st, (double)ca, q.mg);
     }
@@ -199,6 +221,16 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
         pc += w;
     }
     (void)mrg::draw(S6);  // MakeTransition(runif())
+    if (q.deplete) {
+        // the jSR over the interval before the event, with this interval's open set
+        const double c0 = q.jsr[s];
+        double geff = 0.0;
+        for (int i = 0; i < n; ++i)
+            if (q.state[off + i] == 1) geff += q.gch * (1.0 - (double)cav[i] / c0);
+        const double c1 = jsr_advance(q, c0, geff, dtw);
+        q.jsr[s] = c1;
+        if (c1 < q.jsr_min[s]) q.jsr_min[s] = c1;
+    }
     if (q.trace) {
         // IndChannelCREAnalyzer.cpp:78-84: bins up to the next record's time
         // take the current count
@@ -235,7 +267,7 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
 
 // InitOpening of every site (QuasyStationaryReleaseSite.cpp:91-97).
 __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t *__restrict__ rng0,
-                          const uint32_t *__restrict__ jump) {
+                          const uint32_t *__restrict__ jump, const double *__restrict__ jsr0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= q.S) return;
     const int off = q.off[s], n = q.off[s + 1] - off;
@@ -248,6 +280,8 @@ __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t 
     for (int k = 0; k < 6; ++k) q.rng[s * 6 + k] = o6[k];
     q.t[s] = 0.0;
     q.cur_bin[s] = 0;
+    q.jsr[s] = jsr0[s];
+    q.jsr_min[s] = jsr0[s];
     q.active[s] = q.t_max > 0.0;
     q.waiting[s] = 0;
     double *st = q.stats + (size_t)s * QS_NSTAT;
@@ -316,6 +350,7 @@ struct CasimQS {
     std::vector<int> h_off, h_init;
     std::vector<uint32_t> h_rng0;
     int *d_init = nullptr, *d_count = nullptr;
+    double *d_jsr0 = nullptr;
     uint32_t *d_rng0 = nullptr, *d_jump = nullptr;
     std::vector<void *> bufs;
     void use() { HIPCK(hipSetDevice(dev)); }
@@ -383,10 +418,13 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         int *off = h->alloc<int>(S + 1);
         float *dx = h->alloc<float>(h->nch), *dy = h->alloc<float>(h->nch);
         double *jsr = h->alloc<double>(S);
+        h->d_jsr0 = h->alloc<double>(S);
+        q.jsr_min = h->alloc<double>(S);
         HIPCK(hipMemcpyAsync(off, h->h_off.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, h->stream));
         HIPCK(hipMemcpyAsync(dx, x.data(), sizeof(float) * h->nch, hipMemcpyHostToDevice, h->stream));
         HIPCK(hipMemcpyAsync(dy, y.data(), sizeof(float) * h->nch, hipMemcpyHostToDevice, h->stream));
         HIPCK(hipMemcpyAsync(jsr, c->jsr, sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
+        HIPCK(hipMemcpyAsync(h->d_jsr0, c->jsr, sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
         q.off = off;
         q.x = dx;
         q.y = dy;
@@ -407,6 +445,20 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         for (int k = 0; k < NPAR; ++k) q.par[k] = c->params[k];
         q.t_max = c->t_max;
         q.mg = c->mg;
+        q.deplete = c->jsr_deplete != 0;
+        if (q.deplete) {
+            REQUIRE(c->jsr_volume > 0 && c->element_volume > 0 && c->jsr_tau_refill > 0 && c->conductance >= 0 &&
+                        c->jsr_cq_total >= 0 && c->jsr_k_cq >= 0 && c->jsr_cq_sites >= 0,
+                    CASIM_E_INVALID, "bad jSR pool parameters");
+            for (int k = 0; k < S; ++k) REQUIRE(c->jsr[k] > 0, CASIM_E_INVALID, "a depleting jSR must start above 0");
+        }
+        q.Vr = q.deplete ? c->element_volume / c->jsr_volume : 0.0;
+        q.cq = c->jsr_cq_total;
+        q.ncq = c->jsr_cq_sites;
+        q.kcq = c->jsr_k_cq;
+        q.tau = q.deplete ? c->jsr_tau_refill : 1.0;
+        q.nsr = c->nsr;
+        q.gch = c->conductance;
         q.thresh = (float)c->spread_threshold;
         h->d_init = h->alloc<int>(S);
         h->d_count = h->alloc<int>(4);
@@ -453,7 +505,7 @@ int casim_qs_reset(CasimQS *h) {
         h->use();
         if (h->q.trace) HIPCK(hipMemsetAsync(h->q.trace, 0xFF, (size_t)h->q.bins * h->q.S, h->stream));
         hipLaunchKernelGGL(k_qs_init, dim3((h->q.S + 127) / 128), dim3(128), 0, h->stream, h->q, h->d_init, h->d_rng0,
-                           h->d_jump);
+                           h->d_jump, (const double *)h->d_jsr0);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(h->stream));
     });
@@ -612,6 +664,17 @@ int casim_qs_get_features(CasimQS *h, float *out) {
         hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(out, h->q.X, sizeof(float) * (size_t)h->nch * QS_F, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int casim_qs_get_jsr(CasimQS *h, double *now, double *lowest) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        h->use();
+        if (now) HIPCK(hipMemcpyAsync(now, h->q.jsr, sizeof(double) * h->q.S, hipMemcpyDeviceToHost, h->stream));
+        if (lowest)
+            HIPCK(hipMemcpyAsync(lowest, h->q.jsr_min, sizeof(double) * h->q.S, hipMemcpyDeviceToHost, h->stream));
         HIPCK(hipStreamSynchronize(h->stream));
     });
 }

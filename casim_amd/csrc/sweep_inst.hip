@@ -69,7 +69,8 @@ void sweep2_launch(CasimDyad *d, double dt) {
     hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL>), dim3(grid), dim3(d->sweep_threads), lds, d->stream,
                        (const T *)d->g[d->cur], (T *)d->g[d->cur ^ 1], (const T *)d->f[d->cur],
                        (T *)d->f[d->cur ^ 1], d->d_off, d->d_snode, d->d_sperm, d->d_sJ, d->d_state, d->d_type,
-                       d->d_jsr, d->d_mouths, d->Ch, d->P, dt, chunk, nchunks, units);
+                       d->d_jsr, (const double *)(d->Q.on ? d->Q.jnew : nullptr), d->d_mouths, d->Ch, d->P, dt,
+                       chunk, nchunks, units);
 }
 
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY>

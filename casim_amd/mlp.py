@@ -32,6 +32,7 @@ def _declare(lib):
         "casim_qs_get_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "casim_qs_get_features": (C.c_int, [P, _fp]),
         "casim_qs_get_window_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "casim_qs_get_jsr": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "casim_qs_record_open_counts": (C.c_int, [P, C.c_double, C.c_int32]),
         "casim_qs_get_open_counts": (C.c_int, [P, C.c_void_p]),
         "casim_qs_get_mouths": (C.c_int, [P, _fp, _fp]),

@@ -24,6 +24,14 @@ from . import abi, analysis, config, workloads
 REFERENCE_RYR = {"kopen0": 457000.0, "ncaopen": 2.12, "kopenmax": 800.0, "kclose0": 245.0, "ncaclose": -0.27}
 
 
+def depleting_pool(volume_nm3=1e7):
+    """The depleting jSR of the coupled-dyad configuration: 1 mM start, 10 mM
+    calsequestrin-like buffer with Kd 0.8 mM, refill tau 100 ms, NSR 1 mM;
+    V_jSR as the reference's example input ("V jSR (nm3)": 1e7)."""
+    return config.PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=volume_nm3, jsr_c0=[1000.0],
+                           jsr_cq_sites=[1.0], jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])
+
+
 def workload(n_sites, geometry_seed=2024, kinetics="reference", precision=abi.CASIM_FP64):
     spec = workloads.batched_dyads(n_sites=n_sites, geometry_seed=geometry_seed, precision=precision)
     if kinetics == "reference":
@@ -35,28 +43,40 @@ def workload(n_sites, geometry_seed=2024, kinetics="reference", precision=abi.CA
 BIN_MS = 0.01  # IndChannelCREAnalyzer's time-histogram bin (IndChannelCREAnalyzer.cpp:6-8)
 
 
-def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None):
+def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None, pools=None):
     """Full RD + gating for t_max ms: release stats [S, 6], wall seconds;
-    per-bin open counts in run_plain.counts."""
+    per-bin open counts in run_plain.counts, lowest jSR in run_plain.jsr_low
+    (with a depleting pool)."""
     from .dyad import DyadHIP
     dt = workloads.DT_MS
     steps = int(round(t_max / dt))
     dev = DyadHIP(spec)
     dev.RecordOpenCounts(int(round(BIN_MS / dt)), int(round(t_max / BIN_MS)) + 1)
-    dev.InitOpeningAll()
     ri, rb = spec.rim_arrays()
-    dev.Update(ri, rb, spec.jsr_array([jsr]))
+
+    def start():
+        dev.InitOpeningAll()
+        if pools is not None:
+            dev.UpdatePools(ri, rb)
+        else:
+            dev.Update(ri, rb, spec.jsr_array([jsr]))
+
+    if pools is not None:
+        dev.AttachPools(pools)
+    start()
     dev.Step(dt, min(chunk, steps))  # graph capture outside the timed region
     dev.Reset()
-    dev.InitOpeningAll()
-    dev.Update(ri, rb, spec.jsr_array([jsr]))
+    start()
     dev.Synchronize()
+    jlow = np.full(spec.n_sites, np.inf)
     t0 = time.perf_counter()
     done = 0
     while done < steps:
         k = min(chunk, steps - done)
         dev.Step(dt, k)
         done += k
+        if pools is not None:
+            jlow = np.minimum(jlow, dev.GetPools()[0][0])
         if log and done % (20 * chunk) == 0:
             dev.Synchronize()
             log({"stage": "plain", "steps": done, "of": steps, "s": round(time.perf_counter() - t0, 1)})
@@ -64,15 +84,17 @@ def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None):
     wall = time.perf_counter() - t0
     stats = dev.GetReleaseStats()
     run_plain.counts = dev.GetOpenCounts()
+    run_plain.jsr_low = jlow if pools is not None else None
     dev.close()
     return stats, wall
 
 
-def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None):
+def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None, pools=None):
     """Surrogate-driven quasi-stationary path to t_max: window stats [S, 4],
     event-record stats [S, 6], wall seconds, fallback solves."""
     from .surrogate import QuasiStationarySites
-    qs = QuasiStationarySites(spec, np.full(spec.n_sites, jsr), t_max, net, spread_threshold=threshold)
+    qs = QuasiStationarySites(spec, np.full(spec.n_sites, jsr), t_max, net, spread_threshold=threshold,
+                              pools=pools)
     qs.RecordOpenCounts(BIN_MS, int(round(t_max / BIN_MS)) + 1)
     t0 = time.perf_counter()
     qs.RunSimulation()
@@ -81,6 +103,7 @@ def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None):
     fb = qs.fallback_solves
     info = {"rounds": qs.rounds, "fallback_s": qs.fallback_s}
     run_qs.counts = qs.GetOpenCounts()
+    run_qs.jsr_low = qs.GetJsr()[1] if pools is not None else None
     qs.close()
     if log:
         log({"stage": "qs", "s": round(wall, 2), "events": float(full[:, 5].sum()), "fallback_sites": fb, **info})
@@ -143,6 +166,7 @@ def main(argv=None):
     ap.add_argument("--threshold", type=float, default=0.02)
     ap.add_argument("--jsr", type=float, default=1000.0)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--deplete", action="store_true", help="depleting jSR pool in both paths")
     a = ap.parse_args(argv)
 
     def log(d):
@@ -152,12 +176,17 @@ def main(argv=None):
     spec = workload(a.sites, kinetics=a.kinetics)
     n_ch = np.array([len(s) for s in spec.sites])
     net = mlp.MLP.load(a.weights)
-    win, full, wq, fb = run_qs(spec, net, a.t_max, a.jsr, a.threshold, log)
-    res = {"stage": "result", "sites": a.sites, "t_max_ms": a.t_max, "kinetics": a.kinetics,
+    pools = depleting_pool() if a.deplete else None
+    win, full, wq, fb = run_qs(spec, net, a.t_max, a.jsr, a.threshold, log, pools)
+    res = {"stage": "result", "sites": a.sites, "t_max_ms": a.t_max, "kinetics": a.kinetics, "deplete": a.deplete,
            "qs_wall_s": wq, "qs_dyad_ms_per_s": a.sites * a.t_max / wq, "fallback_sites": fb,
            "events_per_site": float(full[:, 5].mean())}
     if not a.skip_plain:
-        plain, wp = run_plain(spec, a.t_max, a.jsr, log=log)
+        plain, wp = run_plain(spec, a.t_max, a.jsr, log=log, pools=pools)
+        if pools is not None:
+            d, p = analysis.ks_2samp(run_plain.jsr_low, run_qs.jsr_low)
+            res["jsr_lowest"] = {"mean_plain": float(np.mean(run_plain.jsr_low)),
+                                 "mean_qs": float(np.mean(run_qs.jsr_low)), "D": d, "p": p}
         res.update({"plain_wall_s": wp, "plain_dyad_ms_per_s": a.sites * a.t_max / wp,
                     "speedup": wp / wq, "ks": compare(plain, win, n_ch, a.t_max),
                     "mean_open_vs_time": compare_histograms(run_plain.counts, run_qs.counts, plain[:, 0], win[:, 0],

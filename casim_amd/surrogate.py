@@ -168,6 +168,15 @@ class CasimQSConfig(C.Structure):
         ("init_channel", C.POINTER(C.c_int32)),
         ("t_max", C.c_double),
         ("spread_threshold", C.c_double),
+        ("jsr_deplete", C.c_int32),
+        ("jsr_volume", C.c_double),
+        ("element_volume", C.c_double),
+        ("conductance", C.c_double),
+        ("jsr_cq_total", C.c_double),
+        ("jsr_cq_sites", C.c_double),
+        ("jsr_k_cq", C.c_double),
+        ("jsr_tau_refill", C.c_double),
+        ("nsr", C.c_double),
     ]
 
 
@@ -182,7 +191,10 @@ class QuasiStationarySites:
     exceeds `spread_threshold` at any RyR takes its next event with the full
     solver's mouth values instead (RunSimulation's `fallback`)."""
 
-    def __init__(self, spec, jsr_site, t_max, net, spread_threshold=0.02, mg=0.0, rng_states=None):
+    def __init__(self, spec, jsr_site, t_max, net, spread_threshold=0.02, mg=0.0, rng_states=None, pools=None):
+        """pools: a config.PoolSpec with jsr_constant False makes the jSR
+        deplete between events (SRPool's pool, calcium only); jSR starts at
+        jsr_site."""
         self.lib = mlp.lib()
         self.spec, self.net = spec, net
         self.S = spec.n_sites
@@ -201,9 +213,15 @@ class QuasiStationarySites:
         init = np.ascontiguousarray(spec.init_channel or [0] * self.S, dtype=np.int32)
         self.half = (spec.nx - 1) * spec.dx / 2.0
         self._keep = (off, xs, ys, par, self.jsr, rs, init)
+        dep = pools is not None and not pools.jsr_constant
         cfg = CasimQSConfig(self.S, abi.iptr(off), abi.dptr(xs), abi.dptr(ys), self.half, PITCH_NM,
                             config.MODEL_IDS[ct.model], abi.dptr(par), float(mg), abi.dptr(self.jsr),
-                            abi.uptr(rs), abi.iptr(init), float(t_max), float(spread_threshold))
+                            abi.uptr(rs), abi.iptr(init), float(t_max), float(spread_threshold),
+                            int(dep), float(pools.jsr_volume) if dep else 0.0,
+                            config.element_volume(spec), float(ct.conductance[0]),
+                            float(pools.jsr_cq_total) if dep else 0.0, float(pools.jsr_cq_sites[0]) if dep else 0.0,
+                            float(pools.jsr_k_cq[0]) if dep else 0.0, float(pools.jsr_tau_refill[0]) if dep else 1.0,
+                            float(pools.nsr_c[0]) if dep else 0.0)
         h = C.c_void_p()
         abi.check(self.lib.casim_qs_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -273,6 +291,12 @@ class QuasiStationarySites:
         out = np.zeros((int(self.off[-1]), N_FEAT), dtype=np.float32)
         abi.check(self.lib.casim_qs_get_features(self.h, mlp.fptr(out)))
         return out
+
+    def GetJsr(self):
+        """(jSR now [S], lowest jSR reached [S])."""
+        now, low = np.zeros(self.S), np.zeros(self.S)
+        abi.check(self.lib.casim_qs_get_jsr(self.h, abi.dptr(now), abi.dptr(low)))
+        return now, low
 
     def GetMouths(self):
         m = np.zeros(int(self.off[-1]), dtype=np.float32)

@@ -321,7 +321,7 @@ int casim_dyad_get_open_counts(CasimDyad *d, uint8_t *out);
 
 /* Steps per state pass of casim_dyad_step: 2 (the default) advances pairs of
  * steps in one read and one write of the state, where the configuration
- * allows it (plane-marching kernel, no pools attached); 1 forces one pass per
+ * allows it (plane-marching kernel, no cytosol pool); 1 forces one pass per
  * step.  Results do not depend on it.  casim_dyad_steps_per_pass returns the
  * value in effect (1 or 2), -1 for a null handle. */
 int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps);

@@ -95,6 +95,20 @@ typedef struct CasimQSConfig {
     const int32_t *init_channel;        /* [n_sites] */
     double t_max;                       /* ms */
     double spread_threshold;            /* relative ensemble spread that asks for the full solver */
+    /* Depleting jSR (jsr_deplete != 0): SRPool's pool (SRPool.cpp:70-89),
+     * dc/dt = ((nsr - c)/tau - I V_elem / V_jSR) / beta, beta = 1 + n K CQ / (K + c)^2,
+     * advanced over each interval between events with that interval's open
+     * set, I = sum over open RyRs of conductance x (c - c_mouth), c_mouth taken
+     * proportional to c within the interval.  The surrogate sees the current c. */
+    int32_t jsr_deplete;
+    double jsr_volume;                  /* nm^3 */
+    double element_volume;              /* nm^3: the dyad node volume of the currents */
+    double conductance;                 /* (uM/ms)/uM per open RyR (CasimDyadConfig's type conductance) */
+    double jsr_cq_total;                /* uM */
+    double jsr_cq_sites;
+    double jsr_k_cq;                    /* uM */
+    double jsr_tau_refill;              /* ms */
+    double nsr;                         /* uM */
 } CasimQSConfig;
 
 typedef struct CasimQS CasimQS;
@@ -133,6 +147,8 @@ int casim_qs_record_open_counts(CasimQS *q, double bin_ms, int32_t max_bins);
 int casim_qs_get_open_counts(CasimQS *q, uint8_t *out);
 /* Feature rows of every channel for the current open sets [channels x 56]. */
 int casim_qs_get_features(CasimQS *q, float *out);
+/* jSR of every site now and the lowest it reached [S] (either may be NULL). */
+int casim_qs_get_jsr(CasimQS *q, double *now, double *lowest);
 /* Last ensemble mean and relative spread of every channel. */
 int casim_qs_get_mouths(CasimQS *q, float *mean, float *spread);
 

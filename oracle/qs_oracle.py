@@ -45,16 +45,49 @@ def init_opening(n, init_channel, stream):
     return states
 
 
-def event(states, ca, model, params, mg, stream, t):
+class JsrPool:
+    """SRPool's jSR (SRPool.cpp:70-89) for the quasi-stationary path: over an
+    interval with a fixed open set, dc/dt = ((nsr - c)/tau - c geff Vr)/beta,
+    beta = 1 + n K CQ/(K + c)^2, explicit steps of at most 1 us; geff = sum
+    over open RyRs of g (1 - c_mouth/c) at the interval's start (mouth [Ca2+]
+    proportional to c within it), Vr = V_elem / V_jSR."""
+
+    def __init__(self, c0, g, vr, cq, ncq, kcq, tau, nsr):
+        self.c, self.low = float(c0), float(c0)
+        self.g, self.vr, self.cq, self.ncq, self.kcq, self.tau, self.nsr = g, vr, cq, ncq, kcq, tau, nsr
+
+    def advance(self, states, ca, dt):
+        c0 = self.c
+        geff = 0.0
+        for i in range(len(states)):
+            if states[i] == 1:
+                geff += self.g * (1.0 - float(np.float32(ca[i])) / c0)
+        n = float(math.ceil(dt / 1e-3))
+        n = min(max(n, 1.0), 1e6)
+        h = dt / n
+        c = c0
+        for _ in range(int(n)):
+            kc = self.kcq + c
+            beta = 1 + self.ncq * self.kcq * self.cq / (kc * kc)
+            iref = (self.nsr - c) / self.tau
+            c = c + h / beta * (iref - c * geff * self.vr)
+        self.c = c
+        self.low = min(self.low, c)
+
+
+def event(states, ca, model, params, mg, stream, t, pool=None):
     """One Gillespie event (QuasyStationaryReleaseSite.cpp:101-128) with mouth
     [Ca2+] `ca` for the current states; returns (t, chosen channel).  The
-    channel flips (TwoStatePoModel / TwoStateModel::MakeTransition)."""
+    channel flips (TwoStatePoModel / TwoStateModel::MakeTransition).  With a
+    JsrPool the jSR advances over the interval before the flip."""
     n = len(states)
     rates = [oracle.channel_rate(model, params, int(states[i]), float(ca[i]), mg) for i in range(n)]
     total = 0.0
     for r in rates:
         total += r
-    t = t + stream.rexp(1.0 / total)
+    t_start = t
+    dtw = stream.rexp(1.0 / total)
+    t = t + dtw
     u = stream.runif()
     p = 0.0
     idx = 0
@@ -65,20 +98,23 @@ def event(states, ca, model, params, mg, stream, t):
             break
         p += w
     stream.runif()  # MakeTransition(runif())
+    if pool is not None:
+        pool.advance(states, ca, dtw)
     states[idx] ^= 1
     return t, idx
 
 
-def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn):
+def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn, pool=None):
     """Whole RunSimulation of one site: the record [(t, states)] and the
-    stream state after it."""
+    stream state after it.  mouth_fn(states, jsr) gives the mouth [Ca2+]."""
     stream = Stream(rng_state)
     states = init_opening(n, init_channel, stream)
     t = 0.0
     rec = [(t, states.copy())]
     n_open = 1
     while n_open >= 1 and t < t_max:
-        t, idx = event(states, mouth_fn(states), model, params, mg, stream, t)
+        jsr = pool.c if pool is not None else None
+        t, idx = event(states, mouth_fn(states, jsr), model, params, mg, stream, t, pool)
         n_open = int(states.sum())
         rec.append((t, states.copy()))
     return rec, stream.state

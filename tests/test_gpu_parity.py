@@ -330,6 +330,37 @@ def test_pools_jsr_depletion_fp64(gpu, oracle_mod):
     assert np.array_equal(dev.GetRngStates(), ora.dyad.rng_states())
 
 
+def test_two_step_passes_with_jsr_pool(gpu):
+    """A depleting jSR pool (no cytosol pool) keeps paired passes, bit for bit
+    equal to one step per pass: fields, jSR, channels, RNG, currents."""
+    from casim_amd.config import PoolSpec
+    sp = workloads.batched_dyads(n_sites=8, geometry_seed=4)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    ps = PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=2e5, jsr_c0=[1000.0], jsr_cq_sites=[1.0],
+                  jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])
+    ri, rb = sp.rim_arrays()
+    devs = []
+    for steps in (1, 2):
+        d = DyadHIP(sp)
+        d.AttachPools(ps)
+        d.SetStepsPerPass(steps)
+        d.InitOpeningAll()
+        d.UpdatePools(ri, rb)
+        for k in (9, 40, 1, 50):
+            d.Step(1e-5, k)
+        devs.append(d)
+    one, two = devs
+    assert one.StepsPerPass() == 1 and two.StepsPerPass() == 2
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(one.GetField(f), two.GetField(f)), f
+    j1, j2 = one.GetPools()[0], two.GetPools()[0]
+    assert np.array_equal(j1, j2) and j1.min() < 999.9
+    assert np.array_equal(one.GetChannelsStates(), two.GetChannelsStates())
+    assert np.array_equal(one.GetRngStates(), two.GetRngStates())
+    assert np.array_equal(one.GetTotalSRCurrent(), two.GetTotalSRCurrent())
+    assert np.array_equal(one.GetIonsNearChannels(), two.GetIonsNearChannels())
+
+
 def test_pools_reference_test_input_cytosol(gpu, oracle_mod):
     """The reference's test configuration end to end: CytosolPool (leak,
     SERCA, troponin), SRPool with calsequestrin, constant NSR, 2D dyad."""

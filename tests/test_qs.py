@@ -35,7 +35,7 @@ def test_single_channel_closed_form():
     # draws 0 and 1, the only event takes 2 (waiting time), 3 (choice), 4.
     seed = O.stream_states([12345] * 6, 3)[2]
     u, after = O.rng_draws(seed, 5)
-    rec, st = Q.run_site(1, 0, seed, SHEEP, sheep(0.0, 0.0, 500.0), 0.0, 100.0, lambda s: np.full(1, 7.0))
+    rec, st = Q.run_site(1, 0, seed, SHEEP, sheep(0.0, 0.0, 500.0), 0.0, 100.0, lambda s, j: np.full(1, 7.0))
     t = -math.log(u[2]) / 0.5
     assert len(rec) == 2
     assert rec[1][0] == pytest.approx(t, rel=1e-15)
@@ -102,6 +102,25 @@ def test_time_histograms_hand_counts():
     assert h["quarks"].tolist() == [[0, 1], [1, 0], [1, 0]]
     assert h["blips"].tolist() == [[0, 1, 0], [0, 0, 1], [1, 0, 0]]
     assert h["sparks"].tolist() == [[0, 1, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1]]
+
+
+def test_jsr_pool_refill_and_depletion_closed_forms():
+    # no open RyR, no calsequestrin: c_n = nsr + (c0 - nsr)(1 - h/tau)^n with h = 1 us
+    p = Q.JsrPool(500.0, g=1e4, vr=1e-3, cq=0.0, ncq=0.0, kcq=800.0, tau=10.0, nsr=1000.0)
+    p.advance(np.array([0, 0]), np.array([1.0, 1.0]), 0.05)
+    assert math.isclose(p.c, 1000.0 - 500.0 * (1 - 1e-3 / 10.0) ** 50, rel_tol=1e-13)
+    assert p.low == 500.0
+    # one open RyR with mouth [Ca2+] c/4, no refill, no calsequestrin: c_n = c0 (1 - h g 3/4 Vr)^n
+    p = Q.JsrPool(1000.0, g=2e4, vr=1e-3, cq=0.0, ncq=0.0, kcq=800.0, tau=1e300, nsr=0.0)
+    p.advance(np.array([1, 0]), np.array([250.0, 9.0]), 0.0105)  # 11 steps of 0.0105/11 ms
+    h = 0.0105 / 11
+    assert math.isclose(p.c, 1000.0 * (1 - h * 2e4 * 0.75 * 1e-3) ** 11, rel_tol=1e-12)
+    assert p.low == p.c
+    # calsequestrin slows the first step by beta = 1 + n K CQ / (K + c)^2
+    p = Q.JsrPool(1000.0, g=2e4, vr=1e-3, cq=10000.0, ncq=1.0, kcq=800.0, tau=1e300, nsr=0.0)
+    p.advance(np.array([1]), np.array([250.0]), 1e-3)
+    beta = 1 + 800.0 * 10000.0 / 1800.0 ** 2
+    assert math.isclose(p.c, 1000.0 - 1e-3 / beta * 1000.0 * 2e4 * 0.75 * 1e-3, rel_tol=1e-13)
 
 
 def test_waiting_times_are_exponential():
@@ -184,7 +203,7 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
         c = float(ca[_site_rows(qs, s)][0])
         assert np.all(ca[_site_rows(qs, s)] == np.float32(c))
         rec, _ = Q.run_site(n, spec.init_channel[s], spec.rng_states[s], SHEEP, par, 0.0, 2.0,
-                            lambda st: np.full(n, c))
+                            lambda st, j: np.full(n, c))
         want = Q.analyze(rec)
         assert np.array_equal(states[s, :n], rec[-1][1]), s
         np.testing.assert_array_equal(stats[s, [0, 5]], want[[0, 5]])
@@ -194,6 +213,37 @@ def test_constant_surrogate_whole_run_matches_oracle(gpu):
         np.testing.assert_array_equal(win[s, [0, 3]], ww[[0, 3]])
         np.testing.assert_allclose(win[s, 1:3], ww[1:3], rtol=1e-12, atol=1e-15)
     assert stats[:, 5].sum() > 5 * spec.n_sites  # the runs are not trivial
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_depleting_jsr_whole_run_matches_oracle(gpu):
+    """Constant surrogate, depleting jSR between events: channel histories,
+    statistics and the jSR trajectory's end and minimum equal the oracle's."""
+    from casim_amd import surrogate
+    from casim_amd.config import PoolSpec, element_volume
+    spec = _spec(12)
+    jsr = np.full(spec.n_sites, 1000.0)
+    net = _net(zero_bias=0.6)
+    ps = PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=1e6, jsr_c0=[1000.0], jsr_cq_sites=[1.0],
+                  jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])
+    qs = surrogate.QuasiStationarySites(spec, jsr, 2.0, net, spread_threshold=0.02, pools=ps)
+    stats = qs.RunSimulation(fallback=False)
+    now, low = qs.GetJsr()
+    ca = qs.GetMouths()[0]
+    states = qs.GetChannelsStates()
+    par = _par(spec)
+    g = spec.channel_types[0].conductance[0]
+    for s, site in enumerate(spec.sites):
+        n = len(site)
+        c = float(ca[_site_rows(qs, s)][0])
+        pool = Q.JsrPool(1000.0, g, element_volume(spec) / 1e6, 10000.0, 1.0, 800.0, 100.0, 1000.0)
+        rec, _ = Q.run_site(n, spec.init_channel[s], spec.rng_states[s], SHEEP, par, 0.0, 2.0,
+                            lambda st, j: np.full(n, c), pool)
+        assert np.array_equal(states[s, :n], rec[-1][1]), s
+        np.testing.assert_allclose(stats[s], Q.analyze(rec), rtol=1e-12, atol=1e-15)
+        assert now[s] == pytest.approx(pool.c, rel=1e-13) and low[s] == pytest.approx(pool.low, rel=1e-13)
+    assert low.min() < 990.0  # the pool depletes
     qs.close()
 
 
