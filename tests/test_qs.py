@@ -331,3 +331,26 @@ def test_fallback_uses_the_full_solver(gpu):
     for s, site in enumerate(spec.sites):
         np.testing.assert_allclose(used[_site_rows(qs, s)], want[s, :len(site)].astype(np.float32), rtol=1e-6)
     qs.close()
+
+
+@pytest.mark.gpu
+def test_trained_surrogate_beats_linear_superposition(gpu):
+    """What the learned part gains: on held-out geometries the committed
+    ensemble's mouth-[Ca2+] error against the full solver is a fraction of
+    the best distance-only superposition table's (the reference's
+    quasi-stationary model structure, filled from the solver itself)."""
+    import os
+    from casim_amd import linear_baseline, mlp, surrogate
+    from casim_amd.dyad import DyadHIP
+    from casim_amd.train_surrogate import errors
+    w = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "casim_amd",
+                     "surrogate_weights.npz")
+    net = mlp.MLP.load(w)
+    x, y, meta = surrogate.generate(DyadHIP, n_sites=48, seed=10_001)
+    cal = linear_baseline.calibrate(DyadHIP)
+    base = linear_baseline.evaluate(cal, meta)
+    got = errors(net, x, y)
+    print("surrogate", got["rel_mean"], "linear", base["rel_mean"], flush=True)
+    assert got["rel_mean"] < 0.025
+    assert got["rel_mean"] < 0.25 * base["rel_mean"]
+    assert got["rel_mean_above_5uM"] < 0.5 * base["rel_mean_above_5uM"]
