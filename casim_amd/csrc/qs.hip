@@ -43,7 +43,7 @@ struct QsDev {
     int R;         // surrogate rows this round (member stride of out)
     double *stats;
     double *jsr;      // [S] current jSR
-    double *jsr_min;  // [S] lowest jSR reached
+    double *jsr_min;  // [S] lowest jSR over [0, t_max]
     int deplete;
     double Vr, cq, ncq, kcq, tau, nsr, gch;
     float *X;        // [channels x QS_F]
@@ -229,7 +229,13 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
             if (q.state[off + i] == 1) geff += q.gch * (1.0 - (double)cav[i] / c0);
         const double c1 = jsr_advance(q, c0, geff, dtw);
         q.jsr[s] = c1;
-        if (c1 < q.jsr_min[s]) q.jsr_min[s] = c1;
+        // lowest jSR over [0, t_max], what a run of that length observes
+        if (t <= q.t_max) {
+            if (c1 < q.jsr_min[s]) q.jsr_min[s] = c1;
+        } else if (q.t[s] < q.t_max) {
+            const double cw = jsr_advance(q, c0, geff, q.t_max - q.t[s]);
+            if (cw < q.jsr_min[s]) q.jsr_min[s] = cw;
+        }
     }
     if (q.trace) {
         // IndChannelCREAnalyzer.cpp:78-84: bins up to the next record's time

@@ -147,7 +147,8 @@ int casim_qs_record_open_counts(CasimQS *q, double bin_ms, int32_t max_bins);
 int casim_qs_get_open_counts(CasimQS *q, uint8_t *out);
 /* Feature rows of every channel for the current open sets [channels x 56]. */
 int casim_qs_get_features(CasimQS *q, float *out);
-/* jSR of every site now and the lowest it reached [S] (either may be NULL). */
+/* jSR of every site now and the lowest it reached over [0, t_max] [S]
+ * (either may be NULL). */
 int casim_qs_get_jsr(CasimQS *q, double *now, double *lowest);
 /* Last ensemble mean and relative spread of every channel. */
 int casim_qs_get_mouths(CasimQS *q, float *mean, float *spread);

@@ -56,26 +56,33 @@ class JsrPool:
         self.c, self.low = float(c0), float(c0)
         self.g, self.vr, self.cq, self.ncq, self.kcq, self.tau, self.nsr = g, vr, cq, ncq, kcq, tau, nsr
 
-    def advance(self, states, ca, dt):
-        c0 = self.c
-        geff = 0.0
-        for i in range(len(states)):
-            if states[i] == 1:
-                geff += self.g * (1.0 - float(np.float32(ca[i])) / c0)
+    def _run(self, c, geff, dt):
         n = float(math.ceil(dt / 1e-3))
         n = min(max(n, 1.0), 1e6)
         h = dt / n
-        c = c0
         for _ in range(int(n)):
             kc = self.kcq + c
             beta = 1 + self.ncq * self.kcq * self.cq / (kc * kc)
             iref = (self.nsr - c) / self.tau
             c = c + h / beta * (iref - c * geff * self.vr)
-        self.c = c
-        self.low = min(self.low, c)
+        return c
+
+    def advance(self, states, ca, dt, t_prev=0.0, t_max=math.inf):
+        """Advance over dt from t_prev; `low` keeps the lowest value over
+        [0, t_max] (the value at t_max when the interval crosses it)."""
+        c0 = self.c
+        geff = 0.0
+        for i in range(len(states)):
+            if states[i] == 1:
+                geff += self.g * (1.0 - float(np.float32(ca[i])) / c0)
+        self.c = self._run(c0, geff, dt)
+        if t_prev + dt <= t_max:
+            self.low = min(self.low, self.c)
+        elif t_prev < t_max:
+            self.low = min(self.low, self._run(c0, geff, t_max - t_prev))
 
 
-def event(states, ca, model, params, mg, stream, t, pool=None):
+def event(states, ca, model, params, mg, stream, t, pool=None, t_max=math.inf):
     """One Gillespie event (QuasyStationaryReleaseSite.cpp:101-128) with mouth
     [Ca2+] `ca` for the current states; returns (t, chosen channel).  The
     channel flips (TwoStatePoModel / TwoStateModel::MakeTransition).  With a
@@ -99,7 +106,7 @@ def event(states, ca, model, params, mg, stream, t, pool=None):
         p += w
     stream.runif()  # MakeTransition(runif())
     if pool is not None:
-        pool.advance(states, ca, dtw)
+        pool.advance(states, ca, dtw, t_start, t_max)
     states[idx] ^= 1
     return t, idx
 
@@ -114,7 +121,7 @@ def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn, poo
     n_open = 1
     while n_open >= 1 and t < t_max:
         jsr = pool.c if pool is not None else None
-        t, idx = event(states, mouth_fn(states, jsr), model, params, mg, stream, t, pool)
+        t, idx = event(states, mouth_fn(states, jsr), model, params, mg, stream, t, pool, t_max)
         n_open = int(states.sum())
         rec.append((t, states.copy()))
     return rec, stream.state
