@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--qs-tmax", type=float, default=20.0, help="simulated ms of the surrogate leg")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: sites are independent and this build has no multi-GPU path "
+        print("bench.py measures one GPU: sites are independent and this program has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from casim_amd import abi, workloads

@@ -185,8 +185,12 @@ def main(argv=None):
         plain, wp = run_plain(spec, a.t_max, a.jsr, log=log, pools=pools)
         if pools is not None:
             d, p = analysis.ks_2samp(run_plain.jsr_low, run_qs.jsr_low)
+            q = [0.01, 0.1, 0.5, 0.9, 0.99]
             res["jsr_lowest"] = {"mean_plain": float(np.mean(run_plain.jsr_low)),
-                                 "mean_qs": float(np.mean(run_qs.jsr_low)), "D": d, "p": p}
+                                 "mean_qs": float(np.mean(run_qs.jsr_low)), "D": d, "p": p,
+                                 "quantiles": q,
+                                 "plain_q": [float(v) for v in np.quantile(run_plain.jsr_low, q)],
+                                 "qs_q": [float(v) for v in np.quantile(run_qs.jsr_low, q)]}
         res.update({"plain_wall_s": wp, "plain_dyad_ms_per_s": a.sites * a.t_max / wp,
                     "speedup": wp / wq, "ks": compare(plain, win, n_ch, a.t_max),
                     "mean_open_vs_time": compare_histograms(run_plain.counts, run_qs.counts, plain[:, 0], win[:, 0],
