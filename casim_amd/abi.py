@@ -15,6 +15,7 @@ CASIM_MODEL_SHEEP_CANNELL = 1
 CASIM_MODEL_TWO_STATE_MWC = 2
 CASIM_MODEL_TWO_STATE_AHTG = 3
 CASIM_MAX_MODEL_PARAMS = 16
+CASIM_E_INVALID, CASIM_E_DEVICE, CASIM_E_STATE, CASIM_E_RANGE = -1, -2, -3, -4
 CASIM_FP64 = 0
 CASIM_FP32 = 1
 

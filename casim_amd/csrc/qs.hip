@@ -379,10 +379,16 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
                 CASIM_E_INVALID, "bad quasi-stationary configuration");
         REQUIRE(c->model >= 1 && c->model <= 3, CASIM_E_INVALID, "unknown kinetic model");
         REQUIRE(c->pitch > 0 && c->half_extent > 0, CASIM_E_INVALID, "pitch and extent must be > 0");
+        if (c->jsr_deplete) {
+            REQUIRE(c->jsr_volume > 0 && c->element_volume > 0 && c->jsr_tau_refill > 0 && c->conductance >= 0 &&
+                        c->jsr_cq_total >= 0 && c->jsr_k_cq >= 0 && c->jsr_cq_sites >= 0,
+                    CASIM_E_INVALID, "bad jSR pool parameters");
+            for (int k = 0; k < c->n_sites; ++k)
+                REQUIRE(c->jsr[k] > 0, CASIM_E_INVALID, "a depleting jSR must start above 0");
+        }
+        REQUIRE(c->t_max >= 0 && c->spread_threshold >= 0, CASIM_E_INVALID, "t_max and the threshold must be >= 0");
         *out = nullptr;
         h = new CasimQS();
-        HIPCK(hipGetDevice(&h->dev));
-        HIPCK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int S = c->n_sites;
         h->h_off.assign(c->site_channel_offset, c->site_channel_offset + S + 1);
         REQUIRE(h->h_off[0] == 0, CASIM_E_INVALID, "site_channel_offset[0] must be 0");
@@ -402,7 +408,15 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
             REQUIRE(c->rng_state[i] < (i % 6 < 3 ? mrg::M1 : mrg::M2), CASIM_E_INVALID, "bad RNG state");
             h->h_rng0[i] = (uint32_t)c->rng_state[i];
         }
+        for (int k = 0; k < S; ++k) {
+            const uint64_t *r = c->rng_state + (size_t)k * 6;
+            REQUIRE((r[0] | r[1] | r[2]) != 0 && (r[3] | r[4] | r[5]) != 0, CASIM_E_INVALID,
+                    "an RNG state has a zero component triple (RngStream::CheckSeed)");
+        }
         h->h_init.assign(c->init_channel, c->init_channel + S);
+        // host-side checks done: now the device
+        HIPCK(hipGetDevice(&h->dev));
+        HIPCK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         // jump table A^(2p), as the dyad handle builds it
         std::vector<uint32_t> jump((size_t)JUMP_N * 18);
         {
@@ -452,12 +466,6 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.t_max = c->t_max;
         q.mg = c->mg;
         q.deplete = c->jsr_deplete != 0;
-        if (q.deplete) {
-            REQUIRE(c->jsr_volume > 0 && c->element_volume > 0 && c->jsr_tau_refill > 0 && c->conductance >= 0 &&
-                        c->jsr_cq_total >= 0 && c->jsr_k_cq >= 0 && c->jsr_cq_sites >= 0,
-                    CASIM_E_INVALID, "bad jSR pool parameters");
-            for (int k = 0; k < S; ++k) REQUIRE(c->jsr[k] > 0, CASIM_E_INVALID, "a depleting jSR must start above 0");
-        }
         q.Vr = q.deplete ? c->element_volume / c->jsr_volume : 0.0;
         q.cq = c->jsr_cq_total;
         q.ncq = c->jsr_cq_sites;

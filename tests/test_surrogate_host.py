@@ -58,3 +58,52 @@ def test_small_batches_get_plane_chunks():
     assert surrogate.small_batch_tuning(workloads.batched_dyads(n_sites=4096)) is None
     planes, _, _ = surrogate.small_batch_tuning(workloads.batched_dyads(n_sites=16))
     assert planes == 2  # 64 chunks of 79 interior planes
+
+
+def _qs_config(n_sites=2, **over):
+    """A valid quasi-stationary configuration (arrays kept alive in the result)."""
+    import ctypes as C
+    from casim_amd import abi, rng
+    off = np.array([0, 3, 5][:n_sites + 1], dtype=np.int32)
+    xs = np.array([100.0, 131.0, 162.0, 100.0, 131.0])[:off[-1]]
+    ys = np.full(off[-1], 160.0)
+    par = np.zeros(abi.CASIM_MAX_MODEL_PARAMS)
+    par[:5] = [888888.9, 2.0, 1e300, 500.0, 0.0]
+    jsr = np.full(n_sites, 1000.0)
+    rs = np.ascontiguousarray(rng.stream_states(rng.DEFAULT_SEED, n_sites), dtype=np.uint64)
+    init = np.zeros(n_sites, dtype=np.int32)
+    keep = dict(off=off, xs=xs, ys=ys, par=par, jsr=jsr, rs=rs, init=init)
+    vals = dict(n_sites=n_sites, site_channel_offset=abi.iptr(off), channel_x=abi.dptr(xs), channel_y=abi.dptr(ys),
+                half_extent=160.0, pitch=31.0, model=1, params=abi.dptr(par), mg=0.0, jsr=abi.dptr(jsr),
+                rng_state=abi.uptr(rs), init_channel=abi.iptr(init), t_max=1.0, spread_threshold=0.02,
+                jsr_deplete=0, jsr_volume=0.0, element_volume=32.0, conductance=1.0, jsr_cq_total=0.0,
+                jsr_cq_sites=0.0, jsr_k_cq=0.0, jsr_tau_refill=1.0, nsr=0.0)
+    vals.update(over)
+    return surrogate.CasimQSConfig(**vals), keep
+
+
+def test_quasi_stationary_configs_are_validated_on_the_host():
+    import ctypes as C
+    from casim_amd import abi, mlp
+    lib = mlp.lib()
+
+    def create(cfg):
+        h = C.c_void_p()
+        rc = lib.casim_qs_create(C.byref(cfg), C.byref(h))
+        return rc, lib.casim_last_error().decode()
+
+    cfg, keep = _qs_config(model=7)
+    assert create(cfg) == (abi.CASIM_E_INVALID, "unknown kinetic model")
+    cfg, keep = _qs_config(pitch=0.0)
+    assert create(cfg)[0] == abi.CASIM_E_INVALID
+    init = np.array([0, 2], dtype=np.int32)  # site 1 has 2 channels
+    cfg, keep = _qs_config(init_channel=abi.iptr(init))
+    assert create(cfg) == (abi.CASIM_E_RANGE, "init channel out of range")
+    bad = np.array(keep_rs := keep["rs"], dtype=np.uint64)
+    bad[1, 3:] = 0
+    cfg, keep = _qs_config(rng_state=abi.uptr(bad))
+    assert "zero component" in create(cfg)[1]
+    cfg, keep = _qs_config(jsr_deplete=1, jsr_volume=0.0)
+    assert create(cfg) == (abi.CASIM_E_INVALID, "bad jSR pool parameters")
+    assert lib.casim_qs_create(None, None) == abi.CASIM_E_INVALID
+    del keep_rs
