@@ -123,11 +123,11 @@ def surrogate_leg(workloads, n_sites, t_max, plain_value):
     w = os.path.join(ROOT, "casim_amd", "surrogate_weights.npz")
     if not os.path.exists(w):
         return None
-    from casim_amd import mlp, spark_compare
-    net = mlp.MLP.load(w)
+    from casim_amd import mlp, spark_compare, surrogate
+    net, lin = surrogate.load_model(w)
     spec = workloads.batched_dyads(n_sites=n_sites)
-    spark_compare.run_qs(spec, net, 0.05)  # warm-up: workspaces and kernels loaded
-    win, full, wall, fb = spark_compare.run_qs(spec, net, t_max)
+    spark_compare.run_qs(spec, net, 0.05, lin=lin)  # warm-up: workspaces and kernels loaded
+    win, full, wall, fb = spark_compare.run_qs(spec, net, t_max, lin=lin)
     meta = mlp.MLP.meta(w)
     held = meta.get("heldout", {})
     value = n_sites * t_max / wall

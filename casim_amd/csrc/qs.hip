@@ -14,7 +14,8 @@ using namespace casim;
 namespace casim {
 
 constexpr int QS_NEAR = 16;
-constexpr int QS_F = 8 + 3 * QS_NEAR;  // features per RyR
+constexpr int QS_F = 8 + 3 * QS_NEAR;  // geometric features per RyR
+constexpr int QS_FMAX = QS_F + 1;      // + the linear-superposition feature
 constexpr int QS_MAXCH = 128;
 
 // Per-site statistics: the event record's columns (QS_AMP .. QS_EVENTS) and
@@ -46,7 +47,11 @@ struct QsDev {
     double *jsr_min;  // [S] lowest jSR over [0, t_max]
     int deplete;
     double Vr, cq, ncq, kcq, tau, nsr, gch;
-    float *X;        // [channels x QS_F]
+    float *X;        // [channels x nf]
+    int nf;          // features per row: QS_F, or QS_F + 1 with the superposition table
+    const float *lin_tab;  // [lin_n] added [Ca2+] (uM) against r^2 (lattice units^2), at jSR lin_jsr
+    int lin_n;
+    float lin_bg, lin_jsr;
     float *out;      // [members x channels]
     float *ca;       // [channels] mouth [Ca2+] in use
     float *spread;   // [channels]
@@ -59,6 +64,16 @@ struct QsDev {
     int members;
     float ym, ys;
 };
+
+// Added [Ca2+] of one open RyR at squared distance r2 (lattice units): the
+// table, linearly interpolated in r^2, decaying as 1/r past its end.
+__device__ __forceinline__ float lin_added(const QsDev &q, float r2) {
+    const float top = (float)(q.lin_n - 1);
+    if (r2 >= top) return q.lin_tab[q.lin_n - 1] * sqrtf(top / r2);
+    const int k = (int)r2;
+    const float w = r2 - (float)k;
+    return q.lin_tab[k] + w * (q.lin_tab[k + 1] - q.lin_tab[k]);
+}
 
 // Features of every channel of every active site: lanes over channels, the
 // 16 nearest open channels kept by insertion (stable for equal distances,
@@ -79,7 +94,7 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
     int nopen = 0;
     for (int j = 0; j < n; ++j) nopen += so[j];
     for (int i = threadIdx.x; i < n; i += 64) {
-        float *f = q.X + (size_t)(base + i) * QS_F;
+        float *f = q.X + (size_t)(base + i) * q.nf;
         const float xi = sx[i], yi = sy[i];
         float rk[QS_NEAR], dxk[QS_NEAR], dyk[QS_NEAR];
 #pragma unroll
@@ -88,13 +103,15 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
             dxk[k] = 0.f;
             dyk[k] = 0.f;
         }
-        float sinv = 0.f, sexp = 0.f;
+        float sinv = 0.f, sexp = 0.f, lin = 0.f;
         for (int j = 0; j < n; ++j) {
             if (!so[j] || j == i) continue;
             const float dx = sx[j] - xi, dy = sy[j] - yi;
-            const float r = sqrtf(dx * dx + dy * dy);
+            const float r2 = dx * dx + dy * dy;
+            const float r = sqrtf(r2);
             sinv += 1.f / r;
             sexp += expf(-r);
+            if (q.lin_n) lin += lin_added(q, r2);
             if (r < rk[QS_NEAR - 1]) {
                 rk[QS_NEAR - 1] = r;
                 dxk[QS_NEAR - 1] = dx;
@@ -123,6 +140,11 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
             f[8 + 3 * k] = ok ? dxk[k] : 0.f;
             f[9 + 3 * k] = ok ? dyk[k] : 0.f;
             f[10 + 3 * k] = ok ? 1.f / rk[k] : 0.f;
+        }
+        if (q.lin_n) {
+            // ln of the superposition estimate: background + jSR-scaled sum over open RyRs (self included)
+            if (so[i]) lin += q.lin_tab[0];
+            f[QS_F] = logf(q.lin_bg + (float)(q.jsr[s] / (double)q.lin_jsr) * lin);
         }
     }
 }
@@ -456,7 +478,10 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.active = h->alloc<int>(S);
         q.waiting = h->alloc<int>(S);
         q.stats = h->alloc<double>((size_t)S * QS_NSTAT);
-        q.X = h->alloc<float>((size_t)h->nch * QS_F);
+        q.X = h->alloc<float>((size_t)h->nch * QS_FMAX);
+        q.nf = QS_F;
+        q.lin_tab = nullptr;
+        q.lin_n = 0;
         q.ca = h->alloc<float>(h->nch);
         q.spread = h->alloc<float>(h->nch);
         q.ovr = h->alloc<float>(h->nch);
@@ -508,6 +533,7 @@ int casim_qs_destroy(CasimQS *h) {
     for (void *p : h->bufs) (void)hipFree(p);
     if (h->q.out) (void)hipFree(h->q.out);
     if (h->q.trace) (void)hipFree(h->q.trace);
+    if (h->q.lin_tab) (void)hipFree((void *)h->q.lin_tab);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CASIM_OK;
@@ -529,7 +555,9 @@ int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch
                  int32_t *n_active, int32_t *n_waiting) {
     return guard([&] {
         REQUIRE(h && m, CASIM_E_INVALID, "null handle");
-        REQUIRE(mlp_inputs(m) == QS_F, CASIM_E_INVALID, "the surrogate must take 56 features");
+        REQUIRE(mlp_inputs(m) == h->q.nf, CASIM_E_INVALID,
+                h->q.lin_n ? "with a superposition table the surrogate must take 57 features"
+                           : "the surrogate must take 56 features (57 with a superposition table)");
         h->use();
         const int E = mlp_members(m);
         if (!h->q.out || h->q.members != E) {
@@ -622,6 +650,31 @@ int casim_qs_get_stats(CasimQS *h, double *out) {
     });
 }
 
+int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float background, float jsr_ref) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        REQUIRE(n == 0 || (added && n >= 2 && background > 0 && jsr_ref > 0), CASIM_E_INVALID,
+                "a superposition table needs >= 2 entries, background > 0 and jsr_ref > 0");
+        h->use();
+        HIPCK(hipStreamSynchronize(h->stream));
+        if (h->q.lin_tab) (void)hipFree((void *)h->q.lin_tab);
+        h->q.lin_tab = nullptr;
+        h->q.lin_n = 0;
+        h->q.nf = QS_F;
+        if (n > 0) {
+            float *t = nullptr;
+            HIPCK(hipMalloc(&t, sizeof(float) * n));
+            HIPCK(hipMemcpyAsync(t, added, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+            HIPCK(hipStreamSynchronize(h->stream));
+            h->q.lin_tab = t;
+            h->q.lin_n = n;
+            h->q.lin_bg = background;
+            h->q.lin_jsr = jsr_ref;
+            h->q.nf = QS_F + 1;
+        }
+    });
+}
+
 int casim_qs_record_open_counts(CasimQS *h, double bin_ms, int32_t max_bins) {
     return guard([&] {
         REQUIRE(h, CASIM_E_INVALID, "null handle");
@@ -677,7 +730,8 @@ int casim_qs_get_features(CasimQS *h, float *out) {
         HIPCK(hipGetLastError());
         hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
         HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(out, h->q.X, sizeof(float) * (size_t)h->nch * QS_F, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(hipMemcpyAsync(out, h->q.X, sizeof(float) * (size_t)h->nch * h->q.nf, hipMemcpyDeviceToHost,
+                             h->stream));
         HIPCK(hipStreamSynchronize(h->stream));
     });
 }

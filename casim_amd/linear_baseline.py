@@ -51,6 +51,15 @@ def calibrate(DyadHIP, jsr=JSR_CAL, steps=surrogate.QS_STEPS):
     return {"r2": r2, "added": added, "background": bg, "jsr": jsr}
 
 
+def feature_table(cal):
+    """The calibration as a dense LinearFeature table over integer r^2
+    (linear interpolation between the lattice distances it was read at)."""
+    from .surrogate import LinearFeature
+    top = int(cal["r2"][-1])
+    dense = np.interp(np.arange(top + 1), cal["r2"], cal["added"])
+    return LinearFeature(dense, float(cal["background"]), float(cal["jsr"]))
+
+
 def predict(cal, xy_nm, open_mask, jsr_uM):
     """Mouth [Ca2+] of every RyR of one site by superposition."""
     xy = np.asarray(xy_nm, dtype=np.float64) / surrogate.PITCH_NM
@@ -93,14 +102,14 @@ def main(argv=None):
         print(json.dumps(d), flush=True)
 
     from .dyad import DyadHIP
-    from . import mlp
     cal = calibrate(DyadHIP)
     log({"stage": "calibrated", "background_uM": cal["background"],
          "added_uM": {int(r): float(v) for r, v in zip(cal["r2"][:8], cal["added"][:8])}})
-    xh, yh, meta = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000)
+    net, lin = surrogate.load_model(a.weights)
+    xh, yh, meta = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, lin=lin)
     log({"stage": "linear_superposition", **evaluate(cal, meta)})
     from .train_surrogate import errors
-    log({"stage": "surrogate", **errors(mlp.MLP.load(a.weights), xh, yh)})
+    log({"stage": "surrogate", **errors(net, xh, yh)})
     return 0
 
 

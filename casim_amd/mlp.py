@@ -31,6 +31,7 @@ def _declare(lib):
         "casim_qs_set_mouths": (C.c_int, [P, C.c_int32, _fp]),
         "casim_qs_get_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "casim_qs_get_features": (C.c_int, [P, _fp]),
+        "casim_qs_set_linear_feature": (C.c_int, [P, _fp, C.c_int32, C.c_float, C.c_float]),
         "casim_qs_get_window_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "casim_qs_get_jsr": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "casim_qs_record_open_counts": (C.c_int, [P, C.c_double, C.c_int32]),
@@ -123,15 +124,16 @@ class MLP:
         abi.check(self.lib.casim_mlp_predict(self.h, fptr(x), n, fptr(mean), fptr(spread), fptr(raw)))
         return mean, spread, raw
 
-    def save(self, path, meta=None):
-        """Shape, normalisation, every member's FP32 parameters and an optional
-        JSON-able description (training data, held-out errors) (.npz)."""
+    def save(self, path, meta=None, **extra):
+        """Shape, normalisation, every member's FP32 parameters, an optional
+        JSON-able description (training data, held-out errors) and extra
+        arrays (.npz)."""
         import json
         m, s, ym, ys = getattr(self, "norm", (np.zeros(self.n_in), np.ones(self.n_in), 0.0, 1.0))
         np.savez(path, shape=np.array([self.n_in, self.width, self.depth, self.members]),
                  in_mean=m, in_std=s, out=np.array([ym, ys]),
                  params=np.stack([self.params(e) for e in range(self.members)]),
-                 meta=np.array(json.dumps(meta or {})))
+                 meta=np.array(json.dumps(meta or {})), **extra)
 
     @staticmethod
     def meta(path):

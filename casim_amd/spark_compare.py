@@ -89,12 +89,12 @@ def run_plain(spec, t_max, jsr=1000.0, chunk=1000, log=None, pools=None):
     return stats, wall
 
 
-def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None, pools=None):
+def run_qs(spec, net, t_max, jsr=1000.0, threshold=0.02, log=None, pools=None, lin=None):
     """Surrogate-driven quasi-stationary path to t_max: window stats [S, 4],
     event-record stats [S, 6], wall seconds, fallback solves."""
     from .surrogate import QuasiStationarySites
     qs = QuasiStationarySites(spec, np.full(spec.n_sites, jsr), t_max, net, spread_threshold=threshold,
-                              pools=pools)
+                              pools=pools, lin=lin)
     qs.RecordOpenCounts(BIN_MS, int(round(t_max / BIN_MS)) + 1)
     t0 = time.perf_counter()
     qs.RunSimulation()
@@ -172,12 +172,12 @@ def main(argv=None):
     def log(d):
         print(json.dumps(d), flush=True)
 
-    from . import mlp
+    from . import surrogate
     spec = workload(a.sites, kinetics=a.kinetics)
     n_ch = np.array([len(s) for s in spec.sites])
-    net = mlp.MLP.load(a.weights)
+    net, lin = surrogate.load_model(a.weights)
     pools = depleting_pool() if a.deplete else None
-    win, full, wq, fb = run_qs(spec, net, a.t_max, a.jsr, a.threshold, log, pools)
+    win, full, wq, fb = run_qs(spec, net, a.t_max, a.jsr, a.threshold, log, pools, lin)
     res = {"stage": "result", "sites": a.sites, "t_max_ms": a.t_max, "kinetics": a.kinetics, "deplete": a.deplete,
            "qs_wall_s": wq, "qs_dyad_ms_per_s": a.sites * a.t_max / wq, "fallback_sites": fb,
            "events_per_site": float(full[:, 5].mean())}

@@ -16,6 +16,9 @@ Features of RyR i (lattice units of 31 nm, cleft centre at the origin):
   5,6  position of i relative to the cleft centre      7  distance of i to the rim
   8 + 3k .. 10 + 3k  (dx, dy, 1/r) to the k-th nearest open RyR j != i,
                      k < 16, zeros when fewer are open
+  56 (with a LinearFeature)  ln of the linear-superposition estimate of the
+                     mouth [Ca2+]: background + jSR/jSR_ref x sum over open RyRs
+                     (i included) of the single-channel table at r^2
 """
 import ctypes as C
 import dataclasses
@@ -31,12 +34,40 @@ QS_STEPS = 50_000
 DT_MS = 1e-5
 
 
-def site_features(xy_nm, open_mask, jsr_uM, half_extent_nm):
-    """Feature rows [n, N_FEAT] of one site (see the module docstring)."""
+@dataclasses.dataclass
+class LinearFeature:
+    """Single-channel superposition table for feature 56: added [Ca2+] (uM)
+    at integer r^2 (lattice units^2) 0..len-1, interpolated linearly in r^2,
+    decaying as 1/r past the end; at jSR jsr_ref."""
+    added: np.ndarray
+    background: float
+    jsr_ref: float
+
+    def at(self, r2):
+        r2 = np.asarray(r2, dtype=np.float64)
+        top = len(self.added) - 1
+        k = np.minimum(np.floor(r2).astype(np.int64), top - 1)
+        w = r2 - k
+        inside = self.added[k] + w * (self.added[k + 1] - self.added[k])
+        far = self.added[top] * np.sqrt(top / np.maximum(r2, 1e-30))
+        return np.where(r2 >= top, far, inside)
+
+    def estimate(self, xy_lattice, open_mask, jsr_uM):
+        op = np.asarray(open_mask, dtype=bool)
+        d2 = ((xy_lattice[:, None, :] - xy_lattice[None, op, :]) ** 2).sum(axis=2)
+        g = self.at(d2)
+        return self.background + jsr_uM / self.jsr_ref * g.sum(axis=1)
+
+
+def site_features(xy_nm, open_mask, jsr_uM, half_extent_nm, lin=None):
+    """Feature rows [n, N_FEAT] (N_FEAT + 1 with a LinearFeature) of one site
+    (see the module docstring)."""
     xy = (np.asarray(xy_nm, dtype=np.float64) - half_extent_nm) / PITCH_NM
     n = len(xy)
     op = np.asarray(open_mask, dtype=bool)
-    f = np.zeros((n, N_FEAT))
+    f = np.zeros((n, N_FEAT + (1 if lin is not None else 0)))
+    if lin is not None:
+        f[:, N_FEAT] = np.log(lin.estimate(xy, op, jsr_uM))
     f[:, 0] = op
     f[:, 1] = jsr_uM / 1000.0
     f[:, 2] = op.sum() / 100.0
@@ -63,12 +94,12 @@ def site_features(xy_nm, open_mask, jsr_uM, half_extent_nm):
     return f
 
 
-def batch_features(spec, states, jsr_site):
+def batch_features(spec, states, jsr_site, lin=None):
     half = (spec.nx - 1) * spec.dx / 2.0
     rows = []
     for s, site in enumerate(spec.sites):
         xy = np.array([(c[0], c[1]) for c in site])
-        rows.append(site_features(xy, states[s, :len(site)] == 1, jsr_site[s], half))
+        rows.append(site_features(xy, states[s, :len(site)] == 1, jsr_site[s], half, lin))
     return np.concatenate(rows, axis=0)
 
 
@@ -140,16 +171,27 @@ def qs_solve(DyadHIP, spec, states, jsr_site, steps=QS_STEPS, tuning=None):
     return m
 
 
-def generate(DyadHIP, n_sites=4096, seed=0, steps=QS_STEPS, tuning=None):
+def generate(DyadHIP, n_sites=4096, seed=0, steps=QS_STEPS, tuning=None, lin=None):
     """One batch of training samples: (features [rows, F], ln Ca [rows], meta)."""
     spec = workloads.batched_dyads(n_sites=n_sites, geometry_seed=10_000 + seed, clamped=True)
     gen = np.random.default_rng(20_000 + seed)
     states = random_open_sets(spec, gen)
     jsr = gen.uniform(200.0, 1200.0, n_sites)
     mouth = qs_solve(DyadHIP, spec, states, jsr, steps, tuning)
-    x = batch_features(spec, states, jsr)
+    x = batch_features(spec, states, jsr, lin)
     y = np.concatenate([np.log(mouth[s, :len(site)]) for s, site in enumerate(spec.sites)])
     return x.astype(np.float32), y.astype(np.float32), {"spec": spec, "states": states, "jsr": jsr, "mouth": mouth}
+
+
+def load_model(path):
+    """(ensemble, LinearFeature or None) from a file written by
+    train_surrogate (mlp.MLP.save with the superposition table)."""
+    net = mlp.MLP.load(path)
+    z = np.load(path, allow_pickle=False)
+    lin = None
+    if "lin_added" in z:
+        lin = LinearFeature(np.asarray(z["lin_added"], dtype=np.float64), float(z["lin_bg"]), float(z["lin_jsr"]))
+    return net, lin
 
 
 class CasimQSConfig(C.Structure):
@@ -191,7 +233,8 @@ class QuasiStationarySites:
     exceeds `spread_threshold` at any RyR takes its next event with the full
     solver's mouth values instead (RunSimulation's `fallback`)."""
 
-    def __init__(self, spec, jsr_site, t_max, net, spread_threshold=0.02, mg=0.0, rng_states=None, pools=None):
+    def __init__(self, spec, jsr_site, t_max, net, spread_threshold=0.02, mg=0.0, rng_states=None, pools=None,
+                 lin=None):
         """pools: a config.PoolSpec with jsr_constant False makes the jSR
         deplete between events (SRPool's pool, calcium only); jSR starts at
         jsr_site."""
@@ -225,6 +268,11 @@ class QuasiStationarySites:
         h = C.c_void_p()
         abi.check(self.lib.casim_qs_create(C.byref(cfg), C.byref(h)))
         self.h = h
+        self.lin = lin
+        if lin is not None:
+            tab = np.ascontiguousarray(lin.added, dtype=np.float32)
+            abi.check(self.lib.casim_qs_set_linear_feature(self.h, mlp.fptr(tab), len(tab), float(lin.background),
+                                                           float(lin.jsr_ref)))
         self.fallback_solves = 0
         self.fallback_s = 0.0
         self.rounds = 0
@@ -288,7 +336,7 @@ class QuasiStationarySites:
         return out
 
     def GetFeatures(self):
-        out = np.zeros((int(self.off[-1]), N_FEAT), dtype=np.float32)
+        out = np.zeros((int(self.off[-1]), N_FEAT + (1 if self.lin is not None else 0)), dtype=np.float32)
         abi.check(self.lib.casim_qs_get_features(self.h, mlp.fptr(out)))
         return out
 

@@ -17,11 +17,11 @@ import numpy as np
 from . import mlp, surrogate
 
 
-def dataset(DyadHIP, batches, sites, seed0=0, tuning=None, log=None):
+def dataset(DyadHIP, batches, sites, seed0=0, tuning=None, log=None, lin=None):
     xs, ys = [], []
     for b in range(batches):
         t0 = time.time()
-        x, y, _ = surrogate.generate(DyadHIP, n_sites=sites, seed=seed0 + b, tuning=tuning)
+        x, y, _ = surrogate.generate(DyadHIP, n_sites=sites, seed=seed0 + b, tuning=tuning, lin=lin)
         xs.append(x)
         ys.append(y)
         if log:
@@ -83,6 +83,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=None)
     ap.add_argument("--data", default=None, help="save / reuse the generated training set (.npz)")
+    ap.add_argument("--no-linear-feature", action="store_true",
+                    help="56 geometric features only (no superposition estimate as feature 56)")
     a = ap.parse_args(argv)
 
     def log(d):
@@ -93,27 +95,41 @@ def main(argv=None):
     spec = workloads.batched_dyads(n_sites=a.sites, clamped=True)
     t = autotune.lookup(spec)
     tuning = (t["planes"], t["threads"], t.get("prefetch", 1)) if t else None
+    lin = None
+    if not a.no_linear_feature:
+        from . import linear_baseline
+        cal = linear_baseline.calibrate(DyadHIP)
+        lin = linear_baseline.feature_table(cal)
+        log({"stage": "superposition_table", "entries": len(lin.added), "background_uM": lin.background})
     if a.data:
         try:
             z = np.load(a.data, allow_pickle=False)
             x, y = z["x"], z["y"]
             log({"stage": "data", "loaded": a.data, "rows": int(len(y))})
         except FileNotFoundError:
-            x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log)
+            x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log, lin)
             np.savez(a.data, x=x, y=y)
     else:
-        x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log)
-    xh, yh, _ = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, tuning=tuning)
+        x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log, lin)
+    xh, yh, meta_h = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, tuning=tuning, lin=lin)
     net = fit(x, y, a.width, a.depth, a.members, a.epochs, a.batch, a.lr, log=log)
     held = errors(net, xh, yh)
     fit_err = errors(net, x[:200_000], y[:200_000])
     log({"stage": "heldout", **held})
     log({"stage": "train_fit", **fit_err})
+    if lin is not None:
+        # the superposition estimate alone on the same held-out mouths
+        true = np.exp(yh.astype(np.float64))
+        rel = np.abs(np.exp(xh[:, surrogate.N_FEAT].astype(np.float64)) - true) / true
+        log({"stage": "heldout_superposition_feature", "rel_mean": float(rel.mean()),
+             "rel_p50": float(np.median(rel)), "rel_p95": float(np.quantile(rel, 0.95))})
     if a.out:
+        extra = {} if lin is None else {"lin_added": lin.added, "lin_bg": np.array(lin.background),
+                                        "lin_jsr": np.array(lin.jsr_ref)}
         net.save(a.out, {"target": "ln mouth [Ca2+] after %d clamped steps of 10 ns from rest" % surrogate.QS_STEPS,
                          "training": "%d batches x %d sites, %d rows; %d epochs, batch %d, Adam lr %g step-decayed"
                                      % (a.batches, a.sites, len(y), a.epochs, a.batch, a.lr),
-                         "heldout": held, "train_fit": fit_err})
+                         "features": x.shape[1], "heldout": held, "train_fit": fit_err}, **extra)
         log({"stage": "saved", "path": a.out})
     return 0
 

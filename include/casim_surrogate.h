@@ -137,6 +137,14 @@ int casim_qs_get_stats(CasimQS *q, double *out);
  * integral of N dt and N, all over [0, t_max] [S x 4]; what a fixed-step run
  * of length t_max reports (casim_dyad_get_release_stats). */
 int casim_qs_get_window_stats(CasimQS *q, double *out);
+/* Adds a 57th feature, ln of the linear-superposition estimate of each
+ * mouth's [Ca2+] (the structure of StationaryDiffusionModel::GetConcentrations,
+ * StationaryDiffusionModel.cpp:3-11): background + (jSR / jsr_ref) x sum over
+ * open RyRs (itself included) of added[r^2], r^2 in squared lattice units,
+ * interpolated linearly in r^2 and decaying as 1/r past the table's end.
+ * n = 0 removes it (56 features).  The surrogate must take the count in use. */
+int casim_qs_set_linear_feature(CasimQS *q, const float *added, int32_t n, float background, float jsr_ref);
+
 /* Time-histogram input of IndChannelCREAnalyzer::RunAnalysis
  * (IndChannelCREAnalyzer.cpp:69-117): at each event at time t, bins from the
  * first unwritten one up to floor(t / bin_ms) (at most max_bins - 1) take the
@@ -145,7 +153,7 @@ int casim_qs_get_window_stats(CasimQS *q, double *out);
 int casim_qs_record_open_counts(CasimQS *q, double bin_ms, int32_t max_bins);
 /* The recorded counts, [max_bins][S] bytes. */
 int casim_qs_get_open_counts(CasimQS *q, uint8_t *out);
-/* Feature rows of every channel for the current open sets [channels x 56]. */
+/* Feature rows of every channel for the current open sets [channels x 56 or 57]. */
 int casim_qs_get_features(CasimQS *q, float *out);
 /* jSR of every site now and the lowest it reached over [0, t_max] [S]
  * (either may be NULL). */

@@ -185,6 +185,24 @@ def test_device_features_match_numpy(gpu):
 
 
 @pytest.mark.gpu
+def test_device_superposition_feature_matches_numpy(gpu):
+    from casim_amd import mlp, surrogate
+    spec = _spec(32)
+    jsr = np.linspace(300.0, 1100.0, spec.n_sites)
+    lin = surrogate.LinearFeature(np.array([320.0, 5.0, 3.0, 2.2, 1.8, 1.5, 1.3]), 0.1, 1000.0)
+    net = mlp.MLP(surrogate.N_FEAT + 1, width=64, depth=2, members=3, seed=5)
+    net.set_normalization(np.zeros(surrogate.N_FEAT + 1), np.full(surrogate.N_FEAT + 1, 4.0), math.log(15.0), 0.5)
+    qs = surrogate.QuasiStationarySites(spec, jsr, 5.0, net, spread_threshold=10.0, lin=lin)
+    for _ in range(4):
+        qs.Round(2)
+        dev = qs.GetFeatures()
+        ref = surrogate.batch_features(spec, qs.GetChannelsStates(), jsr, lin)
+        assert dev.shape == ref.shape == (dev.shape[0], surrogate.N_FEAT + 1)
+        np.testing.assert_allclose(dev, ref, rtol=3e-6, atol=3e-6)
+    qs.close()
+
+
+@pytest.mark.gpu
 def test_constant_surrogate_whole_run_matches_oracle(gpu):
     from casim_amd import surrogate
     spec = _spec(24)
@@ -340,13 +358,13 @@ def test_trained_surrogate_beats_linear_superposition(gpu):
     the best distance-only superposition table's (the reference's
     quasi-stationary model structure, filled from the solver itself)."""
     import os
-    from casim_amd import linear_baseline, mlp, surrogate
+    from casim_amd import linear_baseline, surrogate
     from casim_amd.dyad import DyadHIP
     from casim_amd.train_surrogate import errors
     w = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "casim_amd",
                      "surrogate_weights.npz")
-    net = mlp.MLP.load(w)
-    x, y, meta = surrogate.generate(DyadHIP, n_sites=48, seed=10_001)
+    net, lin = surrogate.load_model(w)
+    x, y, meta = surrogate.generate(DyadHIP, n_sites=48, seed=10_001, lin=lin)
     cal = linear_baseline.calibrate(DyadHIP)
     base = linear_baseline.evaluate(cal, meta)
     got = errors(net, x, y)

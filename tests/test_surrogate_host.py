@@ -107,3 +107,15 @@ def test_quasi_stationary_configs_are_validated_on_the_host():
     assert create(cfg) == (abi.CASIM_E_INVALID, "bad jSR pool parameters")
     assert lib.casim_qs_create(None, None) == abi.CASIM_E_INVALID
     del keep_rs
+
+
+def test_superposition_feature_table_rules():
+    lin = surrogate.LinearFeature(np.array([300.0, 5.0, 3.0, 2.0, 1.0]), 0.1, 1000.0)
+    assert np.allclose(lin.at([0, 1, 1.5, 4, 16]), [300.0, 5.0, 4.0, 1.0, 0.5])  # 16: 1 * sqrt(4/16)
+    xy = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 2.0]])
+    est = lin.estimate(xy, [True, False, True], 500.0)
+    # open: 0 and 2; r^2 0-2 = 4, 1-0 = 1, 1-2 = 5 (beyond: 1 * sqrt(4/5))
+    want = 0.1 + 0.5 * np.array([300.0 + 1.0, 5.0 + np.sqrt(4 / 5), 1.0 + 300.0])
+    assert np.allclose(est, want)
+    f = surrogate.site_features(xy * surrogate.PITCH_NM + 160.0, [1, 0, 1], 500.0, 160.0, lin)
+    assert f.shape == (3, surrogate.N_FEAT + 1) and np.allclose(f[:, -1], np.log(want))
