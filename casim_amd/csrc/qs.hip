@@ -51,6 +51,7 @@ struct QsDev {
     int nf;          // features per row: QS_F, or QS_F + 1 with the superposition table
     const float *lin_tab;  // [lin_n] added [Ca2+] (uM) against r^2 (lattice units^2), at jSR lin_jsr
     int lin_n;
+    int lin_residual;  // the ensemble predicts ln [Ca2+] minus feature QS_F
     float lin_bg, lin_jsr;
     float *out;      // [members x channels]
     float *ca;       // [channels] mouth [Ca2+] in use
@@ -188,8 +189,9 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
             sp = 0.f;
         } else {
             float sum = 0.f, sum2 = 0.f;
+            const float res = q.lin_residual ? q.X[(size_t)(base + i) * q.nf + QS_F] : 0.f;
             for (int e = 0; e < q.members; ++e) {
-                const float v = expf(q.out[(size_t)e * q.R + base + i] * q.ys + q.ym);
+                const float v = expf(q.out[(size_t)e * q.R + base + i] * q.ys + q.ym + res);
                 sum += v;
                 sum2 += v * v;
             }
@@ -482,6 +484,7 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.nf = QS_F;
         q.lin_tab = nullptr;
         q.lin_n = 0;
+        q.lin_residual = 0;
         q.ca = h->alloc<float>(h->nch);
         q.spread = h->alloc<float>(h->nch);
         q.ovr = h->alloc<float>(h->nch);
@@ -650,7 +653,8 @@ int casim_qs_get_stats(CasimQS *h, double *out) {
     });
 }
 
-int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float background, float jsr_ref) {
+int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float background, float jsr_ref,
+                                int32_t residual) {
     return guard([&] {
         REQUIRE(h, CASIM_E_INVALID, "null handle");
         REQUIRE(n == 0 || (added && n >= 2 && background > 0 && jsr_ref > 0), CASIM_E_INVALID,
@@ -660,6 +664,7 @@ int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float
         if (h->q.lin_tab) (void)hipFree((void *)h->q.lin_tab);
         h->q.lin_tab = nullptr;
         h->q.lin_n = 0;
+        h->q.lin_residual = 0;
         h->q.nf = QS_F;
         if (n > 0) {
             float *t = nullptr;
@@ -670,6 +675,7 @@ int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float
             h->q.lin_n = n;
             h->q.lin_bg = background;
             h->q.lin_jsr = jsr_ref;
+            h->q.lin_residual = residual != 0;
             h->q.nf = QS_F + 1;
         }
     });

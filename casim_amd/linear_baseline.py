@@ -109,7 +109,7 @@ def main(argv=None):
     xh, yh, meta = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, lin=lin)
     log({"stage": "linear_superposition", **evaluate(cal, meta)})
     from .train_surrogate import errors
-    log({"stage": "surrogate", **errors(net, xh, yh)})
+    log({"stage": "surrogate", **errors(net, xh, yh, lin=lin)})
     return 0
 
 

@@ -42,6 +42,7 @@ class LinearFeature:
     added: np.ndarray
     background: float
     jsr_ref: float
+    residual: bool = False  # the ensemble predicts ln [Ca2+] minus feature 56
 
     def at(self, r2):
         r2 = np.asarray(r2, dtype=np.float64)
@@ -190,8 +191,22 @@ def load_model(path):
     z = np.load(path, allow_pickle=False)
     lin = None
     if "lin_added" in z:
-        lin = LinearFeature(np.asarray(z["lin_added"], dtype=np.float64), float(z["lin_bg"]), float(z["lin_jsr"]))
+        lin = LinearFeature(np.asarray(z["lin_added"], dtype=np.float64), float(z["lin_bg"]), float(z["lin_jsr"]),
+                            bool(z["lin_residual"]) if "lin_residual" in z else False)
     return net, lin
+
+
+def ensemble_mean(net, x, lin=None):
+    """Mouth [Ca2+] of feature rows x: the mean over members of exp(output),
+    plus feature 56 inside the exponent for a residual ensemble; and the
+    relative spread."""
+    _, _, raw = net.predict(x)
+    raw = raw.astype(np.float64)
+    if lin is not None and lin.residual:
+        raw = raw + np.asarray(x, dtype=np.float64)[:, N_FEAT][None, :]
+    v = np.exp(raw)
+    mean = v.mean(axis=0)
+    return mean, v.std(axis=0) / mean
 
 
 class CasimQSConfig(C.Structure):
@@ -272,7 +287,7 @@ class QuasiStationarySites:
         if lin is not None:
             tab = np.ascontiguousarray(lin.added, dtype=np.float32)
             abi.check(self.lib.casim_qs_set_linear_feature(self.h, mlp.fptr(tab), len(tab), float(lin.background),
-                                                           float(lin.jsr_ref)))
+                                                           float(lin.jsr_ref), int(lin.residual)))
         self.fallback_solves = 0
         self.fallback_s = 0.0
         self.rounds = 0

@@ -29,10 +29,10 @@ def dataset(DyadHIP, batches, sites, seed0=0, tuning=None, log=None, lin=None):
     return np.concatenate(xs), np.concatenate(ys)
 
 
-def errors(net, x, y, threshold=0.02):
+def errors(net, x, y, threshold=0.02, lin=None):
     """Relative error of the ensemble mean in concentration units, overall
-    and on the rows whose spread stays under `threshold`."""
-    mean, spread, _ = net.predict(x)
+    and on the rows whose spread stays under `threshold`; y is ln [Ca2+]."""
+    mean, spread = surrogate.ensemble_mean(net, x, lin)
     true = np.exp(y.astype(np.float64))
     rel = np.abs(mean - true) / true
     ok = spread <= threshold
@@ -85,6 +85,8 @@ def main(argv=None):
     ap.add_argument("--data", default=None, help="save / reuse the generated training set (.npz)")
     ap.add_argument("--no-linear-feature", action="store_true",
                     help="56 geometric features only (no superposition estimate as feature 56)")
+    ap.add_argument("--absolute", action="store_true",
+                    help="with the superposition feature, learn ln [Ca2+] itself instead of its residual")
     a = ap.parse_args(argv)
 
     def log(d):
@@ -100,6 +102,7 @@ def main(argv=None):
         from . import linear_baseline
         cal = linear_baseline.calibrate(DyadHIP)
         lin = linear_baseline.feature_table(cal)
+        lin.residual = not a.absolute
         log({"stage": "superposition_table", "entries": len(lin.added), "background_uM": lin.background})
     if a.data:
         try:
@@ -112,9 +115,11 @@ def main(argv=None):
     else:
         x, y = dataset(DyadHIP, a.batches, a.sites, 0, tuning, log, lin)
     xh, yh, meta_h = surrogate.generate(DyadHIP, n_sites=a.heldout_sites, seed=10_000, tuning=tuning, lin=lin)
-    net = fit(x, y, a.width, a.depth, a.members, a.epochs, a.batch, a.lr, log=log)
-    held = errors(net, xh, yh)
-    fit_err = errors(net, x[:200_000], y[:200_000])
+    # residual ensembles learn ln [Ca2+] minus the superposition estimate (feature 56)
+    yt = y - x[:, surrogate.N_FEAT] if lin is not None and lin.residual else y
+    net = fit(x, yt, a.width, a.depth, a.members, a.epochs, a.batch, a.lr, log=log)
+    held = errors(net, xh, yh, lin=lin)
+    fit_err = errors(net, x[:200_000], y[:200_000], lin=lin)
     log({"stage": "heldout", **held})
     log({"stage": "train_fit", **fit_err})
     if lin is not None:
@@ -125,7 +130,7 @@ def main(argv=None):
              "rel_p50": float(np.median(rel)), "rel_p95": float(np.quantile(rel, 0.95))})
     if a.out:
         extra = {} if lin is None else {"lin_added": lin.added, "lin_bg": np.array(lin.background),
-                                        "lin_jsr": np.array(lin.jsr_ref)}
+                                        "lin_jsr": np.array(lin.jsr_ref), "lin_residual": np.array(lin.residual)}
         net.save(a.out, {"target": "ln mouth [Ca2+] after %d clamped steps of 10 ns from rest" % surrogate.QS_STEPS,
                          "training": "%d batches x %d sites, %d rows; %d epochs, batch %d, Adam lr %g step-decayed"
                                      % (a.batches, a.sites, len(y), a.epochs, a.batch, a.lr),

@@ -142,8 +142,11 @@ int casim_qs_get_window_stats(CasimQS *q, double *out);
  * StationaryDiffusionModel.cpp:3-11): background + (jSR / jsr_ref) x sum over
  * open RyRs (itself included) of added[r^2], r^2 in squared lattice units,
  * interpolated linearly in r^2 and decaying as 1/r past the table's end.
- * n = 0 removes it (56 features).  The surrogate must take the count in use. */
-int casim_qs_set_linear_feature(CasimQS *q, const float *added, int32_t n, float background, float jsr_ref);
+ * n = 0 removes it (56 features).  The surrogate must take the count in use.
+ * residual != 0: the ensemble predicts ln [Ca2+] minus that feature, and the
+ * mouth value is exp(prediction + feature). */
+int casim_qs_set_linear_feature(CasimQS *q, const float *added, int32_t n, float background, float jsr_ref,
+                                int32_t residual);
 
 /* Time-histogram input of IndChannelCREAnalyzer::RunAnalysis
  * (IndChannelCREAnalyzer.cpp:69-117): at each event at time t, bins from the

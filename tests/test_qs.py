@@ -200,6 +200,14 @@ def test_device_superposition_feature_matches_numpy(gpu):
         assert dev.shape == ref.shape == (dev.shape[0], surrogate.N_FEAT + 1)
         np.testing.assert_allclose(dev, ref, rtol=3e-6, atol=3e-6)
     qs.close()
+    # a residual ensemble: mouth = exp(output + feature 56), on the first event of every site
+    lin.residual = True
+    qs = surrogate.QuasiStationarySites(spec, jsr, 5.0, net, spread_threshold=10.0, lin=lin)
+    want, _ = surrogate.ensemble_mean(net, qs.GetFeatures(), lin)
+    assert qs.Round(1)[0] == 1
+    used, _ = qs.GetMouths()
+    np.testing.assert_allclose(used, want, rtol=2e-5)
+    qs.close()
 
 
 @pytest.mark.gpu
@@ -367,7 +375,7 @@ def test_trained_surrogate_beats_linear_superposition(gpu):
     x, y, meta = surrogate.generate(DyadHIP, n_sites=48, seed=10_001, lin=lin)
     cal = linear_baseline.calibrate(DyadHIP)
     base = linear_baseline.evaluate(cal, meta)
-    got = errors(net, x, y)
+    got = errors(net, x, y, lin=lin)
     print("surrogate", got["rel_mean"], "linear", base["rel_mean"], flush=True)
     assert got["rel_mean"] < 0.025
     assert got["rel_mean"] < 0.25 * base["rel_mean"]
