@@ -138,6 +138,46 @@ def surrogate_leg(workloads, n_sites, t_max, plain_value):
             "target": meta.get("target")}
 
 
+def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100):
+    """The coupled configuration: 2D clefts of 201 x 201 nodes at h = 2 nm,
+    each inside a 51^3 cytosol box at h = 20 nm (Ca2+, A, B, troponin; box
+    step every 50 dyad steps), a depleting jSR per site, 64 RyRs, dt = 2 ns,
+    FP64.  One step = channels + dyad sweep (+ box coupling)."""
+    sp = workloads.coupled_dyads(n_sites)
+    box = workloads.coupled_box(sp)
+    dev = DyadHIP(sp)
+    dev.AttachPools(workloads.coupled_pools())
+    dev.AttachCytosolBox(box)
+    dev.InitOpeningAll()
+    dev.UpdatePools()
+    dt = sp.meta["dt_ms"]
+    chunk = max(box.substeps, chunk // box.substeps * box.substeps)
+    steps = max(chunk, steps // chunk * chunk)
+    done = 0
+    while done < max(warmup, 1):
+        dev.Step(dt, chunk)
+        done += chunk
+    dev.Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // chunk):
+        dev.Step(dt, chunk)
+    dev.Synchronize()
+    t = time.perf_counter() - t0
+    ms_sweep, ms_chan = dev.StepProfiled(dt, chunk)
+    jsr = dev.GetPools()[0]
+    ca = dev.GetCytosolBox(0)
+    res = {"workload": sp.meta["workload"], "sites": n_sites, "dyad_grid": "%dx%d" % (sp.nx, sp.ny),
+           "box_grid": "%dx%dx%d" % (box.nx, box.ny, box.nz), "box_substeps": box.substeps, "dt_ns": dt * 1e6,
+           "steps": steps, "wall_s": t, "ms_per_step": 1e3 * t / steps,
+           "value": n_sites * steps * dt / t, "unit": "dyad-ms/s",
+           "sweep_kernel_ms": ms_sweep / chunk, "channel_kernel_ms": ms_chan / chunk,
+           "sweep_gbs": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9,
+           "jsr_min_uM": float(jsr.min()), "box_ca_max_uM": float(ca.max()),
+           "finite": bool(np.isfinite(ca).all() and np.isfinite(dev.GetField(0)).all())}
+    dev.close()
+    return res
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per sweep-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, KiB, scaled by the copy-kernel factor)."""
@@ -166,6 +206,11 @@ def main():
                     help="2: pairs of steps per state pass where possible (default); 1: one pass per step")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--qs-tmax", type=float, default=20.0, help="simulated ms of the surrogate leg")
+    ap.add_argument("--no-coupled", action="store_true")
+    ap.add_argument("--coupled-sites", type=int, default=512)
+    ap.add_argument("--coupled-steps", type=int, default=1000)
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="run only the coupled dyad + cytosol-box leg and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: sites are independent and this program has no multi-GPU path "
@@ -174,6 +219,9 @@ def main():
     from casim_amd import abi, workloads
     from casim_amd.dyad import DyadHIP
     abi.load()
+    if a.coupled_only:
+        print(json.dumps(coupled_leg(workloads, DyadHIP, a.coupled_sites, a.coupled_steps, a.warmup)), flush=True)
+        return
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
     tuning = [int(v) for v in a.tuning.split(",")] if a.tuning else None
     r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning,
@@ -190,6 +238,11 @@ def main():
         log("surrogate leg")
         sur = surrogate_leg(workloads, a.sites, a.qs_tmax, r64["dyad_ms_per_s"])
         log("surrogate:", json.dumps(sur))
+    cpl = None
+    if not a.no_coupled:
+        log("coupled leg: %d sites" % a.coupled_sites)
+        cpl = coupled_leg(workloads, DyadHIP, a.coupled_sites, a.coupled_steps, 100)
+        log("coupled:", json.dumps(cpl))
     cpu = None
     if not a.no_cpu:
         log("cpu baseline")
@@ -230,7 +283,8 @@ def main():
             "sweep_kernel_ms": r32["sweep_kernel_ms"]},
         "kernels": {"sweep_ms": r64["sweep_kernel_ms"], "channels_ms": r64["channel_kernel_ms"]},
         "surrogate": sur,
-        "finite": r64["finite"] and (r32 is None or r32["finite"]),
+        "coupled": cpl,
+        "finite": r64["finite"] and (r32 is None or r32["finite"]) and (cpl is None or cpl["finite"]),
     }
     print(json.dumps(out), flush=True)
 

@@ -36,6 +36,17 @@ class CasimPoolConfig(C.Structure):
     ]
 
 
+class CasimCytosolBoxConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("h", C.c_double), ("origin_x", C.c_double), ("origin_y", C.c_double),
+        ("substeps", C.c_int32), ("far_dirichlet", C.c_int32),
+        ("ion_D", _dp), ("n_buffers", C.c_int32), ("buffer_total", _dp), ("n_bound", C.c_int32),
+        ("bound_ion", _ip), ("bound_buffer", _ip), ("bound_D", _dp), ("bound_kon", _dp), ("bound_koff", _dp),
+        ("bound_dyad", _ip), ("serca_jmax", _dp), ("serca_k", _dp),
+    ]
+
+
 class CasimDyadConfig(C.Structure):
     _fields_ = [
         ("dim", C.c_int32),
@@ -89,6 +100,10 @@ def _declare(lib):
         "casim_dyad_attach_pools": (C.c_int, [P, C.POINTER(CasimPoolConfig)]),
         "casim_dyad_get_pools": (C.c_int, [P, _dp, _dp, _dp]),
         "casim_dyad_cytosol_bound": (C.c_int32, [P]),
+        "casim_dyad_attach_cytosol_box": (C.c_int, [P, C.POINTER(CasimCytosolBoxConfig)]),
+        "casim_dyad_cytosol_box_fields": (C.c_int32, [P]),
+        "casim_dyad_get_cytosol_box": (C.c_int, [P, C.c_int32, _dp]),
+        "casim_dyad_set_cytosol_box": (C.c_int, [P, C.c_int32, _dp]),
         "casim_dyad_run_rd": (C.c_int, [P, C.c_double, C.c_int32]),
         "casim_dyad_run_mc": (C.c_int, [P, C.c_double, C.c_int32]),
         "casim_dyad_step": (C.c_int, [P, C.c_double, C.c_int32]),

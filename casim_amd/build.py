@@ -29,6 +29,7 @@ UNITS = [
     ("casim_dyad.hip", []),
     ("mlp.hip", []),
     ("qs.hip", []),
+    ("cytosol_box.hip", []),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=1", "-DCASIM_SWEEP_DIM=3"]),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=1", "-DCASIM_SWEEP_DIM=2"]),
     ("sweep_inst.hip", ["-DCASIM_SWEEP_FP64=0", "-DCASIM_SWEEP_DIM=3"]),

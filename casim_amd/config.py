@@ -374,3 +374,89 @@ def pools_from_json(j, spec):
         nsr_c=[float(nsr[n]["Concentration (uM)"]) if n in nsr else 0.0 for n in names],
         cytosol_pool=cy.get("Type") == "Pool", cyto_leak_mult=float(cy.get("Leak mult", 0.0)),
         cyto_buffers=bufs, serca=serca)
+
+
+@dataclass
+class CytosolBoxSpec:
+    """3D cytosol box around every site, coupled at the dyad's rim
+    (CasimCytosolBoxConfig).  Box buffers named like a dyad buffer exchange
+    with the dyad's bound field of the same ion at the rim; the dyad's mobile
+    bound fields all need such a partner.  origin: where dyad node (0, 0)
+    sits on the box's z = 0 face, nm."""
+    nx: int
+    ny: int
+    nz: int
+    h: float
+    origin: Tuple[float, float]
+    substeps: int
+    buffers: List[Buffer]
+    far_dirichlet: bool = True
+    ion_D: Optional[Dict[str, float]] = None  # default: the dyad's
+    serca: Dict[str, Tuple[float, float]] = field(default_factory=dict)  # ion -> (Jmax uM/ms, K uM)
+
+    @staticmethod
+    def centred(spec, nx, ny, nz, h, substeps, buffers, **kw):
+        """A box with the dyad centred on its z = 0 face."""
+        ox = ((nx - 1) * h - (spec.nx - 1) * spec.dx) / 2.0
+        oy = ((ny - 1) * h - (spec.ny - 1) * spec.dy) / 2.0
+        return CytosolBoxSpec(nx, ny, nz, h, (ox, oy), substeps, buffers, **kw)
+
+    def bound_fields(self, spec):
+        """(ion index, box buffer index, D, kon, koff, dyad bound field or -1), ion-major."""
+        dyad = {(spec.ions[i].name, spec.buffers[u].name): k for k, (i, u, *_r) in enumerate(spec.bound_fields())}
+        out = []
+        for i, ion in enumerate(spec.ions):
+            for u, b in enumerate(self.buffers):
+                if ion.name in b.kinetics:
+                    D, kon, koff = b.kinetics[ion.name]
+                    out.append((i, u, D, kon, koff, dyad.get((ion.name, b.name), -1)))
+        return out
+
+    def field_names(self, spec):
+        return [i.name for i in spec.ions] + [spec.ions[i].name + "-" + self.buffers[u].name
+                                              for i, u, *_r in self.bound_fields(spec)]
+
+    def nodes(self):
+        return self.nx * self.ny * self.nz
+
+    def node_weights(self):
+        """Finite-volume weight of each node, [z, y, x] in units of h^3: 1/2 per
+        no-flux face the node lies on (what the box's mirror stencil conserves)."""
+        def axis(n):
+            w = np.ones(n)
+            w[0] = w[-1] = 0.5
+            return w
+        return axis(self.nz)[:, None, None] * axis(self.ny)[None, :, None] * axis(self.nx)[None, None, :]
+
+    def to_struct(self, spec):
+        keep = {}
+
+        def arr(name, values, dtype):
+            a = np.ascontiguousarray(np.array(values, dtype=dtype).reshape(-1))
+            if a.size == 0:
+                a = np.zeros(1, dtype=dtype)
+            keep[name] = a
+            return a
+
+        bf = self.bound_fields(spec)
+        names = [i.name for i in spec.ions]
+        D = [(self.ion_D or {}).get(n, spec.ions[k].D) for k, n in enumerate(names)]
+        c = abi.CasimCytosolBoxConfig()
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        c.h = self.h
+        c.origin_x, c.origin_y = float(self.origin[0]), float(self.origin[1])
+        c.substeps = int(self.substeps)
+        c.far_dirichlet = int(self.far_dirichlet)
+        c.ion_D = abi.dptr(arr("iD", D, np.float64))
+        c.n_buffers = len(self.buffers)
+        c.buffer_total = abi.dptr(arr("bt", [b.total for b in self.buffers], np.float64))
+        c.n_bound = len(bf)
+        c.bound_ion = abi.iptr(arr("bi", [r[0] for r in bf], np.int32))
+        c.bound_buffer = abi.iptr(arr("bb", [r[1] for r in bf], np.int32))
+        c.bound_D = abi.dptr(arr("bD", [r[2] for r in bf], np.float64))
+        c.bound_kon = abi.dptr(arr("bkon", [r[3] for r in bf], np.float64))
+        c.bound_koff = abi.dptr(arr("bkoff", [r[4] for r in bf], np.float64))
+        c.bound_dyad = abi.iptr(arr("bd", [r[5] for r in bf], np.int32))
+        c.serca_jmax = abi.dptr(arr("sj", [self.serca.get(n, (0.0, 0.0))[0] for n in names], np.float64))
+        c.serca_k = abi.dptr(arr("sk", [self.serca.get(n, (0.0, 0.0))[1] for n in names], np.float64))
+        return c, keep

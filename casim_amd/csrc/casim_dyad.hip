@@ -29,7 +29,7 @@ std::string &last_error() {
 }
 }  // namespace casim
 
-namespace {
+namespace casim {
 
 // GaussElimination of DyadRDCPU.cpp:163-193 on the equilibrium system of one
 // buffer (:53-81): row t: sum_I kon_t C_t B_I + koff_t B_t = kon_t C_t Btot.
@@ -61,6 +61,10 @@ std::vector<double> buffer_equilibrium(const std::vector<double> &kon, const std
         if (where[i] != -1) ans[i] = a[where[i]][m] / a[where[i]][i];
     return ans;
 }
+
+}  // namespace casim
+
+namespace {
 
 void check_seed(const uint64_t *s) {
     for (int i = 0; i < 3; ++i) REQUIRE(s[i] < mrg::M1, CASIM_E_INVALID, "RNG state component >= m1");
@@ -124,7 +128,7 @@ void CasimDyad::launch_step(double dt) {
 
 bool CasimDyad::pair_ok() const {
     // a cytosol pool moves the Dirichlet rim every step: one step per pass
-    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || (Q.on && Q.cyto)) return false;
+    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || (Q.on && Q.cyto) || box) return false;
     return 4 * (size_t)P.n_grid * P.ny * P.nz * tsize <= 64 * 1024;
 }
 
@@ -483,6 +487,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     (void)hipSetDevice(d->dev);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     d->clear_graphs();
+    casim::box_free(d);
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
                     d->d_sperm, d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
                     d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats, d->Ch.trace};
@@ -520,6 +525,7 @@ int casim_dyad_reset(CasimDyad *d) {
         }
         HIPCK(hipMemcpyAsync(d->d_rng, d->d_rng0, sizeof(uint32_t) * P.S * 6, hipMemcpyDeviceToDevice, d->stream));
         HIPCK(hipStreamSynchronize(d->stream));
+        if (d->box) casim::box_reset(d);
         d->cur = 0;
         d->have_update = false;
     });
@@ -556,12 +562,14 @@ int casim_dyad_update(CasimDyad *d, const double *rim_ions, const double *rim_bo
     return guard([&] {
         REQUIRE(d, CASIM_E_INVALID, "null handle");
         const bool pool_rim = d->Q.on && d->Q.cyto;
-        REQUIRE(pool_rim || (rim_ions && (rim_bound || d->P.nb == 0)), CASIM_E_INVALID, "null rim argument");
+        const bool box_rim = d->box != nullptr;
+        REQUIRE(pool_rim || box_rim || (rim_ions && (rim_bound || d->P.nb == 0)), CASIM_E_INVALID,
+                "null rim argument");
         REQUIRE(d->Q.on || jsr, CASIM_E_INVALID, "null jsr argument");
         d->use();
         const Params &P = d->P;
         d->voltage = voltage;
-        if (!pool_rim) {
+        if (!pool_rim && !box_rim) {
             HIPCK(hipMemcpyAsync(d->d_rim_ions, rim_ions, sizeof(double) * P.ni * P.S, hipMemcpyHostToDevice,
                                  d->stream));
             if (P.nb)
@@ -572,6 +580,8 @@ int casim_dyad_update(CasimDyad *d, const double *rim_ions, const double *rim_bo
             HIPCK(hipMemcpyAsync(d->d_jsr, jsr, sizeof(double) * P.ni * P.S * d->M, hipMemcpyHostToDevice, d->stream));
         if (pool_rim)
             d->launch_rim(d->Q.cions, d->Q.cbb);
+        else if (box_rim)
+            casim::box_rim(d);
         else
             d->launch_rim(d->d_rim_ions, d->d_rim_bound);
         d->launch_efflux();
@@ -597,6 +607,7 @@ int casim_dyad_attach_pools(CasimDyad *d, const CasimPoolConfig *pc) {
     return guard([&] {
         REQUIRE(d && pc, CASIM_E_INVALID, "null argument");
         REQUIRE(!d->Q.on, CASIM_E_STATE, "pools are already attached");
+        REQUIRE(!(d->box && pc->cytosol_pool), CASIM_E_STATE, "a cytosol box is attached");
         d->use();
         const Params &P = d->P;
         Pools Q{};
@@ -731,6 +742,7 @@ int casim_dyad_run_rd(CasimDyad *d, double dt, int32_t site) {
         REQUIRE(d, CASIM_E_INVALID, "null handle");
         REQUIRE(site == -1, CASIM_E_INVALID, "the device path advances every site at once (site = -1)");
         REQUIRE(d->have_update, CASIM_E_STATE, "run_rd before the first update");
+        REQUIRE(!d->box, CASIM_E_STATE, "with a cytosol box, advance with casim_dyad_step");
         d->use();
         d->launch_step(dt);
         HIPCK(hipStreamSynchronize(d->stream));
@@ -753,10 +765,13 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
         REQUIRE(d, CASIM_E_INVALID, "null handle");
         REQUIRE(k_steps >= 1, CASIM_E_INVALID, "k_steps must be >= 1");
         REQUIRE(d->have_update, CASIM_E_STATE, "step before the first update");
+        const int M = casim::box_substeps(d);
+        REQUIRE(!d->box || k_steps % M == 0, CASIM_E_INVALID,
+                "with a cytosol box, k_steps must be a multiple of its substeps");
         d->use();
-        auto key = std::make_tuple((int)k_steps, dt, d->cur);
+        const int cur0 = d->cur, bcur0 = casim::box_cur(d);
+        auto key = std::make_tuple((int)k_steps, dt, cur0 | bcur0 << 1);
         auto it = d->graphs.find(key);
-        const int cur0 = d->cur;
         if (it == d->graphs.end()) {
             HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
             try {
@@ -767,6 +782,7 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                         d->launch_rim(d->Q.cions, d->Q.cbb);
                         d->launch_efflux();
                     }
+                    if (d->box) casim::box_efflux(d, k % M == 0, dt);
                     // step 1 recomputes the currents of the preceding update, so
                     // it does not add to the accumulated totals again
                     d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
@@ -783,6 +799,10 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                         d->launch_step(dt);
                         k += 1;
                     }
+                    if (d->box && k % M == 0) {  // box step on the accumulated effluxes, then the rim
+                        casim::box_step(d, M * dt);
+                        casim::box_rim(d);
+                    }
                 }
                 if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
                 d->launch_efflux();
@@ -791,6 +811,7 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                 hipGraph_t junk;
                 (void)hipStreamEndCapture(d->stream, &junk);
                 d->cur = cur0;
+                casim::box_set_cur(d, bcur0);
                 throw;
             }
             hipGraph_t graph;
@@ -798,12 +819,14 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
             hipGraphExec_t exec;
             HIPCK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             HIPCK(hipGraphDestroy(graph));
-            const int flips = d->cur ^ cur0;
+            const int flips = (d->cur ^ cur0) | (casim::box_cur(d) ^ bcur0) << 1;
             d->cur = cur0;
+            casim::box_set_cur(d, bcur0);
             it = d->graphs.emplace(key, std::make_pair(exec, flips)).first;
         }
         HIPCK(hipGraphLaunch(it->second.first, d->stream));
-        d->cur = cur0 ^ it->second.second;
+        d->cur = cur0 ^ (it->second.second & 1);
+        casim::box_set_cur(d, bcur0 ^ (it->second.second >> 1));
     });
 }
 
@@ -812,6 +835,9 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
         REQUIRE(d && kernel_ms, CASIM_E_INVALID, "null argument");
         REQUIRE(k_steps >= 1, CASIM_E_INVALID, "k_steps must be >= 1");
         REQUIRE(d->have_update, CASIM_E_STATE, "step before the first update");
+        const int M = casim::box_substeps(d);
+        REQUIRE(!d->box || k_steps % M == 0, CASIM_E_INVALID,
+                "with a cytosol box, k_steps must be a multiple of its substeps");
         d->use();
         std::vector<hipEvent_t> ev(4 * (size_t)k_steps);
         for (auto &e : ev) HIPCK(hipEventCreate(&e));
@@ -826,6 +852,7 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                     d->launch_rim(d->Q.cions, d->Q.cbb);
                     d->launch_efflux();
                 }
+                if (d->box) casim::box_efflux(d, k % M == 0, dt);
                 HIPCK(hipEventRecord(ev[4 * k], d->stream));
                 d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) | CH_POOL_UPDATE |
                                        CH_POOL_ADVANCE,
@@ -846,6 +873,10 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                     d->launch_step(dt);
                     HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
                     k += 1;
+                }
+                if (d->box && k % M == 0) {
+                    casim::box_step(d, M * dt);
+                    casim::box_rim(d);
                 }
             }
             if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);

@@ -1,0 +1,570 @@
+// cytosol_box.hip -- a 3D cytosol box around every release site, coupled to
+// the dyad at its rim.
+//
+// The reference's coupled solver (DyadRD2DwCytosol.cu:53-59, 119-241) puts
+// cleft and cytosol on one 2D grid with two spacings and Dirichlet outer
+// edges.  Here each site owns a box of nx*ny*nz nodes at spacing h whose
+// z = 0 face carries the dyad; the two grids meet at the dyad's rim:
+//   rim      dyad rim node <- bilinear interpolation of the box's z = 0 face
+//   efflux   rim link l carries e_l = D (u_in - u_rim) A_l / d_l out of the
+//            dyad, the flux the dyad's explicit step removes through it;
+//            dt e_l accumulates over the box's substeps
+//   deposit  the accumulated amount enters the box nodes with the same
+//            bilinear weights (reciprocal to the rim read), divided by the
+//            node volume h^3 / 2 of the z = 0 face
+// Dyad + box therefore conserve each ion's total exactly in exact arithmetic
+// (no channels, SERCA or Dirichlet faces).
+//
+// Layout in HBM: box fields [field][site][z][y][x] (x fastest), double,
+// two buffers; the accumulator [exchanged field][site][link].  The box steps
+// every `substeps` dyad steps, so its traffic is a small fraction of the
+// dyad's; its kernels are simple thread-per-node stencils.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "dyad_internal.hpp"
+
+namespace casim {
+
+constexpr int BOX_MAXF = MAXI + MAXB;  // box fields
+constexpr int BOX_MAXU = MAXU;         // box buffers
+
+// Everything the box kernels read, passed by value.
+struct BoxArgs {
+    int nx, ny, nz, NB, S;
+    int ni, nbb, nbu, nf;  // box ions, bound fields, buffers, fields (ni + nbb)
+    int ne;                // exchanged fields
+    int far_dirichlet;
+    double ih2, inv_face_volume;
+    double D[BOX_MAXF];
+    int bion[MAXB], bbuf[MAXB];
+    double kon[MAXB], koff[MAXB], total[BOX_MAXU];
+    double jmax[MAXI], K2[MAXI], base[MAXI];  // SERCA: Jmax, K^2, c0^2/(c0^2+K^2)
+    int ex_box[BOX_MAXF];                     // exchanged field e: box field
+    int ex_grid[BOX_MAXF];                    // exchanged field e: dyad grid slot
+    double ex_D[BOX_MAXF];                    // exchanged field e: diffusion in the dyad
+    // dyad side
+    int dnz;            // dyad nodes in z (1 in 2D)
+    size_t dfs;         // dyad grid field stride (S * NS)
+    int dNS;            // dyad nodes per site
+    int ncol, nlink;
+    const int *col_node;    // [ncol] dyad flat node of the column at z = 0
+    const int *col_box;     // [ncol*4] box face nodes
+    const double *col_w;    // [ncol*4] bilinear weights
+    const int *link_rim;    // [nlink] dyad flat node of the rim end (z = 0)
+    const int *link_in;     // [nlink] dyad flat node of the interior end (z = 0)
+    const int *link_col;    // [nlink] rim column
+    const double *link_fac; // [nlink] A / d per unit z weight
+    const double *zw;       // [dnz] z weights (1 in 2D; 1/2 on the two z-planes in 3D)
+    const int *dep_row;     // [nx*ny] deposit row of a z = 0 node, or -1
+    const int *dep_ptr;     // [rows+1]
+    const int *dep_link;    // [entries]
+    const double *dep_w;    // [entries]
+    double *acc;            // [ne][S][nlink]
+};
+
+struct CytoBox {
+    BoxArgs a{};
+    int substeps = 1;
+    int cur = 0;
+    double *b[2] = {nullptr, nullptr};
+    std::vector<double> init;  // [nf] rest values
+    std::vector<void *> owned;
+};
+
+// ---- kernels ---------------------------------------------------------------
+
+__global__ void k_box_init(double *b0, double *b1, const double *__restrict__ init, BoxArgs a) {
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per = (size_t)a.S * a.NB;
+    if (n >= per) return;
+    for (int f = 0; f < a.nf; ++f) b0[f * per + n] = b1[f * per + n] = init[f];
+}
+
+// Dyad rim nodes from the box: one thread per (rim column, site).
+template <typename T>
+__global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs a) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (c >= a.ncol) return;
+    const size_t per = (size_t)a.S * a.NB;
+    const int *q = a.col_box + 4 * c;
+    const double *w = a.col_w + 4 * c;
+    const size_t node = (size_t)s * a.dNS + a.col_node[c];
+    for (int e = 0; e < a.ne; ++e) {
+        const double *u = box + a.ex_box[e] * per + (size_t)s * a.NB;
+        const double v = w[0] * u[q[0]] + w[1] * u[q[1]] + w[2] * u[q[2]] + w[3] * u[q[3]];
+        const size_t o = a.ex_grid[e] * a.dfs + node;
+        for (int z = 0; z < a.dnz; ++z) g0[o + z] = g1[o + z] = (T)v;
+    }
+}
+
+// Rim-link effluxes of the dyad state g, times dt, into the accumulator:
+// one thread per (link, site).
+template <typename T>
+__global__ void k_box_efflux(const T *__restrict__ g, BoxArgs a, double dt, int first) {
+#pragma clang fp contract(off)
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (l >= a.nlink) return;
+    const size_t rim = (size_t)s * a.dNS + a.link_rim[l], in = (size_t)s * a.dNS + a.link_in[l];
+    const double fac = a.link_fac[l];
+    for (int e = 0; e < a.ne; ++e) {
+        const T *u = g + a.ex_grid[e] * a.dfs;
+        double t = 0.0;
+        for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)u[in + z] - (double)u[rim + z]);
+        const double ev = a.ex_D[e] * (fac * t);
+        double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink + l;
+        *acc = first ? dt * ev : *acc + dt * ev;
+    }
+}
+
+// One explicit box step.  Block (64, 4): lanes along x, four y-rows; grid
+// (y-blocks, z, site).  Ions and bound fields sit in separate arrays whose
+// loops run to the compile-time bounds NI, NB with guards, so every array
+// index is a constant and the per-node values stay in registers.
+template <int NI, int NB>
+__global__ __launch_bounds__(256) void k_box_step(const double *__restrict__ bin, double *__restrict__ bout, BoxArgs a,
+                                                  double dt) {
+#pragma clang fp contract(off)
+    const int x = threadIdx.x, y = blockIdx.x * 4 + threadIdx.y, z = blockIdx.y, s = blockIdx.z;
+    if (x >= a.nx || y >= a.ny) return;
+    const int n = x + a.nx * (y + a.ny * z);
+    const size_t per = (size_t)a.S * a.NB;
+    const double *ib = bin + (size_t)s * a.NB;
+    double *ob = bout + (size_t)s * a.NB;
+    if (a.far_dirichlet && (x == 0 || x == a.nx - 1 || y == 0 || y == a.ny - 1 || z == a.nz - 1)) {
+        for (int f = 0; f < a.nf; ++f) ob[f * per + n] = ib[f * per + n];
+        return;
+    }
+    // mirror (no-flux) neighbours on the faces
+    const int sy = a.nx, sz = a.nx * a.ny;
+    const int xl = x > 0 ? n - 1 : n + 1, xr = x < a.nx - 1 ? n + 1 : n - 1;
+    const int yl = y > 0 ? n - sy : n + sy, yr = y < a.ny - 1 ? n + sy : n - sy;
+    const int zl = z > 0 ? n - sz : n + sz, zr = z < a.nz - 1 ? n + sz : n - sz;
+    auto diffuse = [&](int f, double &vo, double &eo) {
+        const double *u = ib + f * per;
+        const double vc = u[n];
+        vo = vc;
+        const double lap = (u[xr] - 2 * vc + u[xl]) * a.ih2 + (u[yr] - 2 * vc + u[yl]) * a.ih2 +
+                           (u[zr] - 2 * vc + u[zl]) * a.ih2;
+        eo = a.D[f] * lap;
+    };
+    double vi[NI], ei[NI], di[NI], vb[NB], eb[NB], db[NB];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        vi[j] = ei[j] = di[j] = 0.0;
+        if (j < a.ni) diffuse(j, vi[j], ei[j]);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        vb[b] = eb[b] = db[b] = 0.0;
+        if (b < a.nbb) diffuse(a.ni + b, vb[b], eb[b]);
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        if (j >= a.ni) continue;
+        double R = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (b >= a.nbb || a.bion[b] != j) continue;
+            // free buffer: total minus its bound fields, in bound-field order
+            double fr = a.total[a.bbuf[b]];
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < a.nbb && a.bbuf[c] == a.bbuf[b]) fr -= vb[c];
+            const double R1 = a.koff[b] * vb[b] - a.kon[b] * vi[j] * fr;
+            eb[b] += -R1;
+            R += R1;
+        }
+        ei[j] += R;
+        if (a.jmax[j] != 0.0) {
+            const double c2 = vi[j] * vi[j];
+            ei[j] -= a.jmax[j] * (c2 / (c2 + a.K2[j]) - a.base[j]);
+        }
+    }
+    if (z == 0) {
+        const int row = a.dep_row[n];
+        if (row >= 0) {
+            const int k0 = a.dep_ptr[row], k1 = a.dep_ptr[row + 1];
+            for (int e = 0; e < a.ne; ++e) {
+                const double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink;
+                double sum = 0.0;
+                for (int k = k0; k < k1; ++k) sum += a.dep_w[k] * acc[a.dep_link[k]];
+                const int f = a.ex_box[e];
+#pragma unroll
+                for (int j = 0; j < NI; ++j)
+                    if (f == j) di[j] = sum;
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    if (f == a.ni + b) db[b] = sum;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+        if (j < a.ni) ob[j * per + n] = vi[j] + ei[j] * dt + di[j] * a.inv_face_volume;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        if (b < a.nbb) ob[(a.ni + b) * per + n] = vb[b] + eb[b] * dt + db[b] * a.inv_face_volume;
+}
+
+// ---- host side -------------------------------------------------------------
+
+namespace {
+
+template <class T>
+T *upload(CytoBox *b, const std::vector<T> &h, hipStream_t st) {
+    void *p = nullptr;
+    HIPCK(hipMalloc(&p, sizeof(T) * (h.empty() ? 1 : h.size())));
+    b->owned.push_back(p);
+    if (!h.empty()) {
+        HIPCK(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+        HIPCK(hipStreamSynchronize(st));  // h may be a temporary
+    }
+    return (T *)p;
+}
+
+}  // namespace
+
+void box_efflux(CasimDyad *d, bool first, double dt) {
+    CytoBox *b = d->box;
+    dim3 grid((b->a.nlink + 127) / 128, d->P.S);
+    if (d->precision == CASIM_FP64)
+        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)d->g[d->cur], b->a,
+                           dt, (int)first);
+    else
+        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)d->g[d->cur], b->a, dt,
+                           (int)first);
+    HIPCK(hipGetLastError());
+}
+
+void box_step(CasimDyad *d, double dt) {
+    CytoBox *b = d->box;
+    dim3 grid((b->a.ny + 3) / 4, b->a.nz, d->P.S), block(64, 4);
+    if (b->a.ni == 1 && b->a.nbb <= 3)
+        hipLaunchKernelGGL((k_box_step<1, 3>), grid, block, 0, d->stream, b->b[b->cur], b->b[b->cur ^ 1], b->a, dt);
+    else
+        hipLaunchKernelGGL((k_box_step<MAXI, MAXB>), grid, block, 0, d->stream, b->b[b->cur], b->b[b->cur ^ 1],
+                           b->a, dt);
+    HIPCK(hipGetLastError());
+    b->cur ^= 1;
+}
+
+void box_rim(CasimDyad *d) {
+    CytoBox *b = d->box;
+    dim3 grid((b->a.ncol + 127) / 128, d->P.S);
+    if (d->precision == CASIM_FP64)
+        hipLaunchKernelGGL(k_box_rim<double>, grid, dim3(128), 0, d->stream, (double *)d->g[0], (double *)d->g[1],
+                           b->b[b->cur], b->a);
+    else
+        hipLaunchKernelGGL(k_box_rim<float>, grid, dim3(128), 0, d->stream, (float *)d->g[0], (float *)d->g[1],
+                           b->b[b->cur], b->a);
+    HIPCK(hipGetLastError());
+}
+
+void box_reset(CasimDyad *d) {
+    CytoBox *b = d->box;
+    const size_t per = (size_t)b->a.S * b->a.NB;
+    double *init = upload(b, b->init, d->stream);
+    hipLaunchKernelGGL(k_box_init, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, d->stream, b->b[0], b->b[1],
+                       init, b->a);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(d->stream));
+    HIPCK(hipFree(init));
+    b->owned.pop_back();
+    b->cur = 0;
+}
+
+void box_free(CasimDyad *d) {
+    if (!d->box) return;
+    for (void *p : d->box->owned)
+        if (p) (void)hipFree(p);
+    delete d->box;
+    d->box = nullptr;
+}
+
+int box_substeps(const CasimDyad *d) { return d->box ? d->box->substeps : 1; }
+int box_cur(const CasimDyad *d) { return d->box ? d->box->cur : 0; }
+void box_set_cur(CasimDyad *d, int cur) {
+    if (d->box) d->box->cur = cur;
+}
+
+}  // namespace casim
+
+using namespace casim;
+
+namespace {
+
+// Bilinear support of the point (px, py) nm on the box's z = 0 face: four
+// face nodes and weights, (x0,y0) (x1,y0) (x0,y1) (x1,y1).  A zero weight
+// points at the node of the other side so that no face node is referenced
+// without need.
+void bilinear(const BoxArgs &a, double h, double px, double py, int q[4], double w[4]) {
+    const double gx = px / h, gy = py / h;
+    const int ax = (int)std::floor(gx), ay = (int)std::floor(gy);
+    const double fx = gx - ax, fy = gy - ay;
+    REQUIRE(ax >= 1 && ay >= 1 && ax + (fx > 0) <= a.nx - 2 && ay + (fy > 0) <= a.ny - 2, CASIM_E_INVALID,
+            "the dyad's rim must lie inside the box's z = 0 face, off its x and y faces");
+    const int x1 = fx > 0 ? ax + 1 : ax, y1 = fy > 0 ? ay + 1 : ay;
+    q[0] = ax + a.nx * ay;
+    q[1] = x1 + a.nx * ay;
+    q[2] = ax + a.nx * y1;
+    q[3] = x1 + a.nx * y1;
+    w[0] = (1 - fx) * (1 - fy);
+    w[1] = fx * (1 - fy);
+    w[2] = (1 - fx) * fy;
+    w[3] = fx * fy;
+}
+
+}  // namespace
+
+extern "C" {
+
+int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) {
+    return guard([&] {
+        REQUIRE(d && c, CASIM_E_INVALID, "null argument");
+        REQUIRE(!d->box, CASIM_E_STATE, "a cytosol box is already attached");
+        REQUIRE(!(d->Q.on && d->Q.cyto), CASIM_E_STATE, "a cytosol pool is attached");
+        REQUIRE(c->nx >= 4 && c->ny >= 4 && c->nz >= 2, CASIM_E_INVALID, "box needs nx, ny >= 4 and nz >= 2");
+        REQUIRE(c->nx <= 64 && c->nz <= 65535, CASIM_E_INVALID, "box needs nx <= 64 (one wave per x-row)");
+        REQUIRE((int64_t)c->nx * c->ny * c->nz < (1LL << 30), CASIM_E_INVALID, "box too large");
+        REQUIRE(c->h > 0 && std::isfinite(c->h), CASIM_E_INVALID, "box spacing must be > 0");
+        REQUIRE(c->substeps >= 1, CASIM_E_INVALID, "substeps must be >= 1");
+        REQUIRE(c->ion_D, CASIM_E_INVALID, "null ion_D");
+        REQUIRE(c->n_buffers >= 0 && c->n_buffers <= BOX_MAXU, CASIM_E_INVALID, "too many box buffers");
+        REQUIRE(c->n_bound >= 0 && c->n_bound <= MAXB, CASIM_E_INVALID, "too many box bound fields");
+        REQUIRE(c->n_bound == 0 || (c->buffer_total && c->bound_ion && c->bound_buffer && c->bound_D && c->bound_kon &&
+                                    c->bound_koff && c->bound_dyad),
+                CASIM_E_INVALID, "null bound-field array");
+        const Params &P = d->P;
+        d->use();
+        auto *b = new CytoBox();
+        try {
+            BoxArgs &a = b->a;
+            a.nx = c->nx;
+            a.ny = c->ny;
+            a.nz = c->nz;
+            a.NB = c->nx * c->ny * c->nz;
+            a.S = P.S;
+            a.ni = P.ni;
+            a.nbb = c->n_bound;
+            a.nbu = c->n_buffers;
+            a.nf = a.ni + a.nbb;
+            a.far_dirichlet = c->far_dirichlet != 0;
+            a.ih2 = 1.0 / (c->h * c->h);
+            a.inv_face_volume = 2.0 / (c->h * c->h * c->h);
+            for (int j = 0; j < P.ni; ++j) {
+                REQUIRE(c->ion_D[j] >= 0, CASIM_E_INVALID, "box ion_D must be >= 0");
+                a.D[j] = c->ion_D[j];
+                const double jm = c->serca_jmax ? c->serca_jmax[j] : 0.0;
+                const double K = c->serca_k ? c->serca_k[j] : 0.0;
+                REQUIRE(jm == 0.0 || K > 0.0, CASIM_E_INVALID, "SERCA needs K > 0");
+                a.jmax[j] = jm;
+                a.K2[j] = K * K;
+                const double c02 = P.c0[j] * P.c0[j];
+                a.base[j] = jm != 0.0 ? c02 / (c02 + a.K2[j]) : 0.0;
+            }
+            for (int u = 0; u < a.nbu; ++u) a.total[u] = c->buffer_total[u];
+            std::vector<int> seen(MAXB, 0);
+            for (int k = 0; k < a.nbb; ++k) {
+                REQUIRE(c->bound_ion[k] >= 0 && c->bound_ion[k] < P.ni, CASIM_E_INVALID, "bad box bound_ion");
+                REQUIRE(c->bound_buffer[k] >= 0 && c->bound_buffer[k] < a.nbu, CASIM_E_INVALID,
+                        "bad box bound_buffer");
+                REQUIRE(c->bound_D[k] >= 0, CASIM_E_INVALID, "box bound_D must be >= 0");
+                a.bion[k] = c->bound_ion[k];
+                a.bbuf[k] = c->bound_buffer[k];
+                a.D[a.ni + k] = c->bound_D[k];
+                a.kon[k] = c->bound_kon[k];
+                a.koff[k] = c->bound_koff[k];
+                const int db = c->bound_dyad[k];
+                REQUIRE(db >= -1 && db < P.nb, CASIM_E_INVALID, "bad box bound_dyad");
+                if (db >= 0) {
+                    REQUIRE(!seen[db], CASIM_E_INVALID, "two box fields exchange with one dyad field");
+                    REQUIRE(P.bion[db] == a.bion[k], CASIM_E_INVALID, "box and dyad bound fields bind different ions");
+                    seen[db] = 1;
+                }
+            }
+            // exchanged fields: every ion, then the dyad's mobile bound fields
+            a.ne = 0;
+            for (int j = 0; j < P.ni; ++j) {
+                a.ex_box[a.ne] = j;
+                a.ex_grid[a.ne] = j;
+                a.ex_D[a.ne] = P.Dion[j];
+                ++a.ne;
+            }
+            for (int db = 0; db < P.nb; ++db) {
+                int k = -1;
+                for (int t = 0; t < a.nbb; ++t)
+                    if (c->bound_dyad[t] == db) k = t;
+                if (P.bface[db] || P.Db[db] == 0.0) continue;  // immobile: its rim value is never read
+                REQUIRE(k >= 0, CASIM_E_INVALID, "a mobile dyad bound field has no box field to exchange with");
+                a.ex_box[a.ne] = a.ni + k;
+                a.ex_grid[a.ne] = P.bslot[db];
+                a.ex_D[a.ne] = P.Db[db];
+                ++a.ne;
+            }
+            // rest state: resting ions, buffers at equilibrium over the ions they bind
+            b->init.assign(a.nf, 0.0);
+            for (int j = 0; j < P.ni; ++j) b->init[j] = P.c0[j];
+            if (d->start_equilibrium) {
+                for (int u = 0; u < a.nbu; ++u) {
+                    std::vector<double> kon, koff, cc;
+                    std::vector<int> idx;
+                    for (int k = 0; k < a.nbb; ++k)
+                        if (a.bbuf[k] == u) {
+                            kon.push_back(a.kon[k]);
+                            koff.push_back(a.koff[k]);
+                            cc.push_back(P.c0[a.bion[k]]);
+                            idx.push_back(k);
+                        }
+                    if (idx.empty()) continue;
+                    auto sol = buffer_equilibrium(kon, koff, cc, a.total[u]);
+                    for (size_t t = 0; t < idx.size(); ++t) b->init[a.ni + idx[t]] = sol[t];
+                }
+            }
+            // dyad rim columns and links
+            const int nx = P.nx, ny = P.ny, nzd = P.nz;
+            a.dnz = P.dim == 3 ? nzd : 1;
+            a.dfs = (size_t)P.S * P.NS;
+            a.dNS = P.NS;
+            a.ncol = 2 * nx + 2 * (ny - 2);
+            a.nlink = 2 * (nx - 2) + 2 * (ny - 2);
+            std::vector<int> col_node(a.ncol), col_box(4 * a.ncol);
+            std::vector<double> col_w(4 * a.ncol);
+            auto column = [&](int cidx, int &x, int &y) {
+                if (cidx < nx) {
+                    x = cidx, y = 0;
+                } else if (cidx < 2 * nx) {
+                    x = cidx - nx, y = ny - 1;
+                } else if (cidx < 2 * nx + ny - 2) {
+                    x = 0, y = cidx - 2 * nx + 1;
+                } else {
+                    x = nx - 1, y = cidx - (2 * nx + ny - 2) + 1;
+                }
+            };
+            auto colof = [&](int x, int y) {
+                if (y == 0) return x;
+                if (y == ny - 1) return nx + x;
+                if (x == 0) return 2 * nx + y - 1;
+                return 2 * nx + ny - 2 + y - 1;
+            };
+            for (int k = 0; k < a.ncol; ++k) {
+                int x, y;
+                column(k, x, y);
+                col_node[k] = (x * ny + y) * nzd;
+                bilinear(a, c->h, c->origin_x + x * d->dx, c->origin_y + y * d->dy, &col_box[4 * k], &col_w[4 * k]);
+            }
+            std::vector<int> link_rim(a.nlink), link_in(a.nlink), link_col(a.nlink);
+            std::vector<double> link_fac(a.nlink);
+            const double thick = P.dim == 3 ? d->dz : d->zext;
+            for (int l = 0; l < a.nlink; ++l) {
+                int x, y, xi, yi;
+                double fac;
+                if (l < 2 * (ny - 2)) {
+                    const int side = l >= ny - 2;
+                    y = 1 + l - side * (ny - 2);
+                    x = side ? nx - 1 : 0;
+                    xi = side ? nx - 2 : 1;
+                    yi = y;
+                    fac = d->dy * thick / d->dx;
+                } else {
+                    const int r = l - 2 * (ny - 2);
+                    const int side = r >= nx - 2;
+                    x = 1 + r - side * (nx - 2);
+                    y = side ? ny - 1 : 0;
+                    yi = side ? ny - 2 : 1;
+                    xi = x;
+                    fac = d->dx * thick / d->dy;
+                }
+                link_rim[l] = (x * ny + y) * nzd;
+                link_in[l] = (xi * ny + yi) * nzd;
+                link_col[l] = colof(x, y);
+                link_fac[l] = fac;
+            }
+            std::vector<double> zw(a.dnz, 1.0);
+            if (P.dim == 3) zw[0] = zw[a.dnz - 1] = 0.5;
+            // deposit rows: face nodes in link order
+            std::vector<std::vector<std::pair<int, double>>> rows(a.nx * a.ny);
+            for (int l = 0; l < a.nlink; ++l) {
+                const int k = link_col[l];
+                for (int q = 0; q < 4; ++q)
+                    if (col_w[4 * k + q] != 0.0) rows[col_box[4 * k + q]].push_back({l, col_w[4 * k + q]});
+            }
+            std::vector<int> dep_row(a.nx * a.ny, -1), dep_ptr(1, 0), dep_link;
+            std::vector<double> dep_w;
+            for (int n = 0; n < a.nx * a.ny; ++n) {
+                if (rows[n].empty()) continue;
+                dep_row[n] = (int)dep_ptr.size() - 1;
+                for (auto &e : rows[n]) {
+                    dep_link.push_back(e.first);
+                    dep_w.push_back(e.second);
+                }
+                dep_ptr.push_back((int)dep_link.size());
+            }
+            a.col_node = upload(b, col_node, d->stream);
+            a.col_box = upload(b, col_box, d->stream);
+            a.col_w = upload(b, col_w, d->stream);
+            a.link_rim = upload(b, link_rim, d->stream);
+            a.link_in = upload(b, link_in, d->stream);
+            a.link_col = upload(b, link_col, d->stream);
+            a.link_fac = upload(b, link_fac, d->stream);
+            a.zw = upload(b, zw, d->stream);
+            a.dep_row = upload(b, dep_row, d->stream);
+            a.dep_ptr = upload(b, dep_ptr, d->stream);
+            a.dep_link = upload(b, dep_link, d->stream);
+            a.dep_w = upload(b, dep_w, d->stream);
+            a.acc = upload(b, std::vector<double>((size_t)a.ne * P.S * a.nlink, 0.0), d->stream);
+            const size_t per = (size_t)a.nf * P.S * a.NB;
+            for (int i = 0; i < 2; ++i) {
+                HIPCK(hipMalloc(&b->b[i], sizeof(double) * per));
+                b->owned.push_back(b->b[i]);
+            }
+            b->substeps = c->substeps;
+            d->box = b;
+            box_reset(d);
+        } catch (...) {
+            if (d->box == b) d->box = nullptr;
+            for (void *p : b->owned)
+                if (p) (void)hipFree(p);
+            delete b;
+            throw;
+        }
+        d->clear_graphs();
+        d->have_update = false;
+    });
+}
+
+int32_t casim_dyad_cytosol_box_fields(const CasimDyad *d) { return d && d->box ? d->box->a.nf : 0; }
+
+int casim_dyad_get_cytosol_box(CasimDyad *d, int32_t field, double *out) {
+    return guard([&] {
+        REQUIRE(d && out, CASIM_E_INVALID, "null argument");
+        REQUIRE(d->box, CASIM_E_STATE, "no cytosol box attached");
+        REQUIRE(field >= 0 && field < d->box->a.nf, CASIM_E_RANGE, "box field out of range");
+        d->use();
+        const size_t per = (size_t)d->P.S * d->box->a.NB;
+        HIPCK(hipMemcpyAsync(out, d->box->b[d->box->cur] + field * per, sizeof(double) * per, hipMemcpyDeviceToHost,
+                             d->stream));
+        HIPCK(hipStreamSynchronize(d->stream));
+    });
+}
+
+int casim_dyad_set_cytosol_box(CasimDyad *d, int32_t field, const double *in) {
+    return guard([&] {
+        REQUIRE(d && in, CASIM_E_INVALID, "null argument");
+        REQUIRE(d->box, CASIM_E_STATE, "no cytosol box attached");
+        REQUIRE(field >= 0 && field < d->box->a.nf, CASIM_E_RANGE, "box field out of range");
+        d->use();
+        const size_t per = (size_t)d->P.S * d->box->a.NB;
+        for (int i = 0; i < 2; ++i)
+            HIPCK(hipMemcpyAsync(d->box->b[i] + field * per, in, sizeof(double) * per, hipMemcpyHostToDevice,
+                                 d->stream));
+        HIPCK(hipStreamSynchronize(d->stream));
+    });
+}
+
+}  // extern "C"

@@ -51,6 +51,14 @@ int guard(F &&f) {
         if (!(cond)) throw casim::Err(code, msg); \
     } while (0)
 
+namespace casim {
+// Equilibrium bound concentrations of one buffer over the ions it binds
+// (GaussElimination of DyadRDCPU.cpp:163-193 on the system of :53-81).
+std::vector<double> buffer_equilibrium(const std::vector<double> &kon, const std::vector<double> &koff,
+                                       const std::vector<double> &c, double total);
+struct CytoBox;  // 3D cytosol box around each site (cytosol_box.hip)
+}  // namespace casim
+
 struct CasimDyad {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -82,7 +90,9 @@ struct CasimDyad {
     int sweep_prefetch = 1;   // planes prefetched beyond x+1
     bool use_sweep = true;
     int steps_per_pass = 2;   // 2: pairs of steps in one k_sweep2 pass where pair_ok()
-    // captured K-step graphs and the buffer flips each makes
+    casim::CytoBox *box = nullptr;  // cytosol box (null when not attached)
+    // captured K-step graphs, keyed by (k_steps, dt, state buffers in use:
+    // bit 0 the dyad's, bit 1 the box's), and the buffer flips each makes
     std::map<std::tuple<int, double, int>, std::pair<hipGraphExec_t, int>> graphs;
 
     size_t grid_elems() const { return (size_t)P.n_grid * P.S * P.NS; }
@@ -112,6 +122,21 @@ void sweep2_launch_f64_3d(CasimDyad *d, double dt);
 void sweep2_launch_f64_2d(CasimDyad *d, double dt);
 void sweep2_launch_f32_3d(CasimDyad *d, double dt);
 void sweep2_launch_f32_2d(CasimDyad *d, double dt);
+}  // namespace casim
+
+namespace casim {
+// Cytosol box launches (cytosol_box.hip), all on the handle's stream.
+// box_efflux: dt times the rim-link effluxes of the current dyad state into
+// the box's accumulator (first = overwrite); box_step: one box step of
+// substeps*dt consuming the accumulator; box_rim: dyad rim nodes from the box.
+void box_efflux(CasimDyad *d, bool first, double dt);
+void box_step(CasimDyad *d, double dt);
+void box_rim(CasimDyad *d);
+void box_reset(CasimDyad *d);
+void box_free(CasimDyad *d);
+int box_substeps(const CasimDyad *d);
+int box_cur(const CasimDyad *d);
+void box_set_cur(CasimDyad *d, int cur);
 }  // namespace casim
 
 struct CasimMLP;

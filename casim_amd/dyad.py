@@ -80,6 +80,26 @@ class DyadHIP:
         return (j.reshape(self.spec.n_ions, -1), ci.reshape(self.spec.n_ions, -1),
                 cb[:nb * self.S].reshape(nb, -1))
 
+    def AttachCytosolBox(self, box):
+        """3D cytosol box around every site, coupled at the rim (config.CytosolBoxSpec)."""
+        self._bc, self._bkeep = box.to_struct(self.spec)
+        self._box = box
+        abi.check(self.lib.casim_dyad_attach_cytosol_box(self.h, C.byref(self._bc)))
+
+    def CytosolBoxFields(self):
+        return int(self.lib.casim_dyad_cytosol_box_fields(self.h))
+
+    def GetCytosolBox(self, f):
+        """Box field f (ions, then the box's bound fields) as [site, z, y, x]."""
+        b = self._box
+        out = np.empty(self.S * b.nz * b.ny * b.nx)
+        abi.check(self.lib.casim_dyad_get_cytosol_box(self.h, f, abi.dptr(out)))
+        return out.reshape(self.S, b.nz, b.ny, b.nx)
+
+    def SetCytosolBox(self, f, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        abi.check(self.lib.casim_dyad_set_cytosol_box(self.h, f, abi.dptr(a)))
+
     def RunRD(self, dt, site=-1):
         abi.check(self.lib.casim_dyad_run_rd(self.h, dt, site))
 

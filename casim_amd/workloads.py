@@ -114,3 +114,52 @@ def batched_dyads(n_sites=4096, seed=rng.DEFAULT_SEED, geometry_seed=2024, n_min
                           % (n_sites, n, n, 64 if precision == 0 else 32, n_min, n_max),
                           "jsr_uM": 1000.0, "dt_ms": DT_MS})
     return spec
+
+
+def troponin():
+    """Immobile troponin-like cytosol buffer, 70 uM (TnC low-affinity site rates)."""
+    return Buffer("Tn", 70.0, {"Calcium": (0.0, 0.0327, 0.0196)})
+
+
+def coupled_dyads(n_sites=512, seed=rng.DEFAULT_SEED, n_ryr=64, extent_nm=400.0, h=2.0, clamped=False,
+                  precision=abi.CASIM_FP64):
+    """2D clefts of 400 x 400 nm at h = 2 nm (201 x 201 nodes, element
+    thickness 16 nm, so a node holds 64 nm^3 as in the 3D cleft), the species
+    of single_dyad with the fixed buffer F on every node, 64 RyRs on an 8 x 8
+    lattice of 31 nm pitch, the channel nearest the centre opened at t = 0,
+    dt 2 ns (diffusion number 0.44 at h = 2 nm).  Pair with coupled_box() and
+    coupled_pools()."""
+    ions, buffers = _species(faces_only_fixed=False)
+    n = int(round(extent_nm / h)) + 1
+    side = int(round(np.sqrt(n_ryr)))
+    c = extent_nm / 2.0
+    lattice = [(c + 31.0 * (i - (side - 1) / 2.0), c + 31.0 * (j - (side - 1) / 2.0), 0)
+               for i in range(side) for j in range(side)]
+    centre = min(range(len(lattice)), key=lambda k: (lattice[k][0] - c) ** 2 + (lattice[k][1] - c) ** 2)
+    spec = DyadSpec(dim=2, nx=n, ny=n, nz=1, dx=h, dy=h, dz=h, z=16.0, radius=1000.0, voltage=0.0,
+                    ions=ions, buffers=buffers, sl={}, start_equilibrium=True,
+                    channel_types=[ryr_type(clamped)], sites=[list(lattice) for _ in range(n_sites)],
+                    rng_states=rng.stream_states(seed, n_sites), precision=precision,
+                    init_channel=[centre] * n_sites,
+                    meta={"workload": "coupled %d dyads 2D %dx%d + cytosol box, %d RyRs"
+                          % (n_sites, n, n, len(lattice)),
+                          "jsr_uM": 1000.0, "dt_ms": 2e-6})
+    return spec
+
+
+def coupled_box(spec, n=51, h=20.0, substeps=50):
+    """The 1 x 1 x 1 um cytosol box at h = 20 nm (51^3 nodes) around each
+    coupled dyad: Ca2+, the dyad's mobile buffers A and B, immobile
+    troponin 70 uM; outer faces held at rest, the dyad centred on the z = 0
+    face; one box step per 50 dyad steps (100 ns, diffusion number 0.33)."""
+    from .config import CytosolBoxSpec
+    mobile = [b for b in spec.buffers if any(k[0] > 0 for k in b.kinetics.values())]
+    return CytosolBoxSpec.centred(spec, n, n, n, h, substeps, mobile + [troponin()], far_dirichlet=True)
+
+
+def coupled_pools():
+    """Depleting jSR: 1 mM start, 10 mM calsequestrin-like buffer with Kd
+    0.8 mM, refill tau 100 ms from a 1 mM NSR, V_jSR 1e7 nm^3."""
+    from .config import PoolSpec
+    return PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=1e7, jsr_c0=[1000.0],
+                    jsr_cq_sites=[1.0], jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])

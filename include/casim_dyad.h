@@ -327,6 +327,67 @@ int casim_dyad_get_open_counts(CasimDyad *d, uint8_t *out);
 int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps);
 int32_t casim_dyad_steps_per_pass(const CasimDyad *d);
 
+/*
+ * 3D cytosol box around each site, coupled at the dyad's rim.  Extension:
+ * the reference's coupled solver (DyadRD2DwCytosol.cu:53-59, 119-241) puts
+ * the cleft and the cytosol on one 2D grid of two spacings; here each site
+ * gets a box of nx*ny*nz nodes at spacing h whose z = 0 face holds the
+ * dyad: dyad node (i, j) sits at (origin_x + i*dx, origin_y + j*dy) on that
+ * face.
+ *   - rim: every rim node of the dyad (all z in 3D) takes, for each field
+ *     that exchanges, the box's value bilinearly interpolated on the z = 0
+ *     face at its position;
+ *   - efflux: each rim link (rim node, its interior neighbour) carries
+ *     D (u_in - u_rim) A / d out of the dyad per unit time (A: the link's
+ *     face area, dy*z or dy*dz per z node with half weight on the two
+ *     z-planes in 3D; d: the link length), exactly what the dyad's explicit
+ *     step removes through that link; dt times it accumulates over the
+ *     substeps and is added to the box nodes with the same bilinear weights,
+ *     divided by the node volume (h^3/2 on the z = 0 face);
+ *   - box step: explicit Euler every `substeps` dyad steps with
+ *     dt_box = substeps * dt; 7-point diffusion with mirror (no-flux) faces,
+ *     buffer reactions koff B - kon C (Btot - sum B), SERCA
+ *     -Jmax (c^2/(c^2+K^2) - c0^2/(c0^2+K^2)) (CytosolPool.cpp:240-250 form);
+ *     far_dirichlet = 1 holds the faces x = 0, nx-1, y = 0, ny-1 and
+ *     z = nz-1 at rest instead.
+ * The box starts at the dyad's resting ion concentrations with its buffers
+ * at equilibrium (or empty when the dyad does not start at equilibrium).
+ * Dyad and box together conserve every ion's total (free + bound) when
+ * channels, SERCA and Dirichlet faces are off.
+ */
+typedef struct CasimCytosolBoxConfig {
+    int32_t nx, ny, nz;             /* box nodes */
+    double h;                       /* box node spacing, nm */
+    double origin_x, origin_y;      /* dyad node (0, 0) on the box's z = 0 face, nm */
+    int32_t substeps;               /* dyad steps per box step, >= 1 */
+    int32_t far_dirichlet;          /* 1: outer faces held at rest; 0: no-flux */
+    const double *ion_D;            /* [n_ions of the dyad] diffusion in the box */
+    int32_t n_buffers;              /* unique box buffers */
+    const double *buffer_total;     /* [n_buffers] */
+    int32_t n_bound;                /* (ion, buffer) bound fields of the box */
+    const int32_t *bound_ion;       /* [n_bound] */
+    const int32_t *bound_buffer;    /* [n_bound] */
+    const double *bound_D;          /* [n_bound] */
+    const double *bound_kon;        /* [n_bound] */
+    const double *bound_koff;       /* [n_bound] */
+    const int32_t *bound_dyad;      /* [n_bound] dyad bound field it exchanges with at the rim, or -1;
+                                       every dyad bound field with D > 0 needs one */
+    const double *serca_jmax;       /* [n_ions] or NULL (0 = no uptake) */
+    const double *serca_k;          /* [n_ions] or NULL */
+} CasimCytosolBoxConfig;
+
+/* Attach a cytosol box to every site (not with a cytosol pool; a jSR pool is
+ * fine).  Afterwards casim_dyad_update takes NULL rims (the box sets them),
+ * casim_dyad_step needs k_steps to be a multiple of substeps and runs one
+ * step per state pass, and casim_dyad_reset also resets the box.  Requires
+ * the bilinear support of every rim node to lie off the box's x and y faces. */
+int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c);
+/* Box fields: the dyad's ions, then the box's bound fields; 0 without a box. */
+int32_t casim_dyad_cytosol_box_fields(const CasimDyad *d);
+/* One box field, [site][z][y][x] (x fastest). */
+int casim_dyad_get_cytosol_box(CasimDyad *d, int32_t field, double *out);
+int casim_dyad_set_cytosol_box(CasimDyad *d, int32_t field, const double *in);
+
 /* HIP stream of the handle (hipStream_t), for callers that time with events. */
 void *casim_dyad_stream(CasimDyad *d);
 

@@ -299,6 +299,11 @@ struct OracleDyad {
     int start_eq;
     double grad_mult, sx, sy;
     double *mult_z;
+    /* extension: per-node rim values set by a cytosol box, [(f*S + s)*ncol + col]
+       for the fields f (ions, then bound) with rim_mask[f] = 1; NULL otherwise */
+    double *rim_nodes;
+    int *rim_mask;
+    int ncol;
 };
 
 static long idx3(const OracleDyad *o, int x, int y, int z, int s) {
@@ -471,7 +476,8 @@ void oracle_destroy(OracleDyad *o) {
                     o->N1, o->K1, o->N2, o->K2, o->type_model, o->type_params, o->type_g, o->ch_off,
                     o->ch_node, o->ch_type, o->ch_state, o->rng0, o->rng, o->ions, o->buffers,
                     o->buf_free, o->evo_ions, o->evo_buffers, o->currents, o->mouths, o->gradients,
-                    o->total_sr_current, o->rim_ions, o->rim_bound, o->jsr, o->mult_z};
+                    o->total_sr_current, o->rim_ions, o->rim_bound, o->jsr, o->mult_z, o->rim_nodes,
+                    o->rim_mask};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(o);
 }
@@ -535,6 +541,15 @@ static void set_mouths(OracleDyad *o) {
 }
 
 /* DyadRDCPU::_SetBoundaries (DyadRDCPU.cpp:290-320): x rims, then y rims. */
+/* Rim column of boundary node (x, y): y = 0 row, y = ny-1 row, x = 0 column,
+ * x = nx-1 column (corners in the rows). */
+static int rim_col(const OracleDyad *o, int x, int y) {
+    if (y == 0) return x;
+    if (y == o->ny - 1) return o->nx + x;
+    if (x == 0) return 2 * o->nx + y - 1;
+    return 2 * o->nx + o->ny - 2 + y - 1;
+}
+
 static void set_boundaries(OracleDyad *o) {
     int S = o->S, ni = o->n_ions, nB = o->n_bound;
     for (int pass = 0; pass < 2; ++pass) {
@@ -545,10 +560,16 @@ static void set_boundaries(OracleDyad *o) {
                     for (int s = 0; s < S; ++s) {
                         int x = pass == 0 ? a : b, y = pass == 0 ? b : a;
                         long id = idx3(o, x, y, z, s);
-                        for (int j = 0; j < ni; ++j) o->ions[id + j * o->NE] = o->rim_ions[s + j * S];
+                        int col = o->rim_nodes ? rim_col(o, x, y) : 0;
+                        for (int j = 0; j < ni; ++j)
+                            o->ions[id + j * o->NE] = o->rim_nodes && o->rim_mask[j]
+                                                          ? o->rim_nodes[((long)j * S + s) * o->ncol + col]
+                                                          : o->rim_ions[s + j * S];
                         for (int i = 0; i < nB; ++i) {
                             /* extension: face-only buffers stay 0 off their planes */
                             double v = buf_total_at(o, o->bound_buf[i], z) == 0.0 ? 0.0 : o->rim_bound[s + i * S];
+                            if (o->rim_nodes && o->rim_mask[ni + i])
+                                v = o->rim_nodes[((long)(ni + i) * S + s) * o->ncol + col];
                             o->buffers[id + i * o->NE] = v;
                         }
                     }
@@ -851,7 +872,15 @@ struct OracleHybrid {
     double *jmax, *kup;
     double *cions, *cbuf, *cfree, *cevo_i, *cevo_b, *cbb; /* cbb: boundary buffers [site + b*S] */
     double *rim_ions, *rim_bound;                         /* constant rim when there is no pool */
+    struct OracleBox *box;                                /* cytosol box, or NULL */
 };
+
+struct OracleBox;
+static void box_to_rim(OracleHybrid *h);
+static void box_link_rates(OracleHybrid *h);
+static void box_after_step(OracleHybrid *h, double dt);
+static void box_reset(struct OracleBox *b);
+static void box_destroy(struct OracleBox *b);
 
 OracleHybrid *oracle_hybrid_create(OracleDyad *o, const CasimPoolConfig *pc, const double *rim_ions,
                                    const double *rim_bound, int ref_indexing) {
@@ -977,6 +1006,7 @@ void oracle_hybrid_destroy(OracleHybrid *h) {
                     h->jmax, h->kup, h->cions, h->cbuf, h->cfree, h->cevo_i, h->cevo_b, h->cbb, h->rim_ions,
                     h->rim_bound};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+    box_destroy(h->box);
     free(h);
 }
 
@@ -1007,6 +1037,7 @@ void oracle_hybrid_reset(OracleHybrid *h) {
         }
     }
     oracle_reset(h->o);
+    if (h->box) box_reset(h->box);
     for (int s = 0; s < S; ++s) {
         int kd = 0;
         for (int j = 0; j < h->ni; ++j) {
@@ -1090,10 +1121,12 @@ static void jsr_run(OracleHybrid *h, double dt, int s) {
 /* ParallelHybridSolver::Update (ParallelHybridSolver.cpp:159-175). */
 void oracle_hybrid_update(OracleHybrid *h) {
     OracleDyad *o = h->o;
+    if (h->box) box_to_rim(h);
     if (h->cyto)
         oracle_update(o, h->cions, h->cbb, h->jnew, o->V);
     else
         oracle_update(o, h->rim_ions, h->rim_bound, h->jnew, o->V);
+    if (h->box) box_link_rates(h);
     /* SRPool::Update (SRPool.cpp:55-68) */
     if (!h->jsr_const) {
         int S = h->S, M = h->M;
@@ -1116,6 +1149,7 @@ void oracle_hybrid_step(OracleHybrid *h, double dt, int k, int n_threads) {
             jsr_run(h, dt, s);
             oracle_run_mc(h->o, dt, s);
         }
+        if (h->box) box_after_step(h, dt);
         oracle_hybrid_update(h);
     }
 }
@@ -1133,3 +1167,351 @@ void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *boun
 }
 
 int oracle_hybrid_cyto_bound(const OracleHybrid *h) { return h->nBc; }
+
+/* ------------------------------------------------------------------------- */
+/* Extension: a 3D cytosol box around each site, coupled at the dyad's rim.
+ * Semantics as declared with CasimCytosolBoxConfig in casim_dyad.h: rim nodes
+ * from the bilinear interpolation of the box's z = 0 face, rim-link effluxes
+ * D (u_in - u_rim) A / d accumulated over `substeps` dyad steps and added to
+ * the box with the same weights, an explicit box step of substeps*dt.  The
+ * reference's coupled solver (DyadRD2DwCytosol.cu:119-241) informs the form
+ * (reaction terms, SERCA relative to rest); the geometry is new, so this
+ * restatement is pinned by conservation, a discrete eigenmode and closed
+ * forms rather than by reference vectors. */
+
+typedef struct OracleBox {
+    int nx, ny, nz, NB, S, ni, nbb, nbu, nf, ne, far, M, count;
+    double h, ih2, ifv;
+    double D[16], kon[8], koff[8], total[8], jmax[4], K2[4], base[4];
+    int bion[8], bbuf[8];
+    int ex_box[16], ex_dyad[16];
+    double ex_D[16];
+    int ncol, nlink, dnz;
+    int *col_x, *col_y, *col_q;
+    double *col_w;
+    int *lk_rx, *lk_ry, *lk_ix, *lk_iy, *lk_col;
+    double *lk_fac, *zw;
+    int *dep_row, *dep_ptr, *dep_link;
+    double *dep_w;
+    double *u, *un, *init, *lrate, *acc;
+} OracleBox;
+
+static void box_destroy(OracleBox *b) {
+    if (!b) return;
+    void *ptrs[] = {b->col_x, b->col_y, b->col_q, b->col_w, b->lk_rx, b->lk_ry, b->lk_ix, b->lk_iy, b->lk_col,
+                    b->lk_fac, b->zw, b->dep_row, b->dep_ptr, b->dep_link, b->dep_w, b->u, b->un, b->init,
+                    b->lrate, b->acc};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+    free(b);
+}
+
+static void box_reset(OracleBox *b) {
+    long per = (long)b->S * b->NB;
+    for (int f = 0; f < b->nf; ++f)
+        for (long n = 0; n < per; ++n) b->u[f * per + n] = b->init[f];
+    b->count = 0;
+}
+
+/* Bilinear support of (px, py) on the z = 0 face; returns -1 when it touches
+ * the x or y faces. */
+static int box_bilinear(const OracleBox *b, double px, double py, int q[4], double w[4]) {
+    double gx = px / b->h, gy = py / b->h;
+    int ax = (int)floor(gx), ay = (int)floor(gy);
+    double fx = gx - ax, fy = gy - ay;
+    if (!(ax >= 1 && ay >= 1 && ax + (fx > 0) <= b->nx - 2 && ay + (fy > 0) <= b->ny - 2)) return -1;
+    int x1 = fx > 0 ? ax + 1 : ax, y1 = fy > 0 ? ay + 1 : ay;
+    q[0] = ax + b->nx * ay;
+    q[1] = x1 + b->nx * ay;
+    q[2] = ax + b->nx * y1;
+    q[3] = x1 + b->nx * y1;
+    w[0] = (1 - fx) * (1 - fy);
+    w[1] = fx * (1 - fy);
+    w[2] = (1 - fx) * fy;
+    w[3] = fx * fy;
+    return 0;
+}
+
+int oracle_hybrid_attach_box(OracleHybrid *h, const CasimCytosolBoxConfig *c) {
+    OracleDyad *o = h->o;
+    if (!c || h->box || h->cyto || c->nx < 4 || c->ny < 4 || c->nz < 2 || c->substeps < 1 || !(c->h > 0)) return -1;
+    if (c->n_bound < 0 || c->n_bound > 8 || c->n_buffers < 0 || c->n_buffers > 8 || o->n_ions > 4) return -1;
+    OracleBox *b = (OracleBox *)xcalloc(1, sizeof *b);
+    int ni = o->n_ions, nB = o->n_bound, S = o->S;
+    b->nx = c->nx;
+    b->ny = c->ny;
+    b->nz = c->nz;
+    b->NB = c->nx * c->ny * c->nz;
+    b->S = S;
+    b->ni = ni;
+    b->nbb = c->n_bound;
+    b->nbu = c->n_buffers;
+    b->nf = ni + b->nbb;
+    b->far = c->far_dirichlet != 0;
+    b->M = c->substeps;
+    b->h = c->h;
+    b->ih2 = 1.0 / (c->h * c->h);
+    b->ifv = 2.0 / (c->h * c->h * c->h);
+    for (int j = 0; j < ni; ++j) {
+        b->D[j] = c->ion_D[j];
+        double jm = c->serca_jmax ? c->serca_jmax[j] : 0.0, K = c->serca_k ? c->serca_k[j] : 0.0;
+        b->jmax[j] = jm;
+        b->K2[j] = K * K;
+        double c02 = o->ion_c0[j] * o->ion_c0[j];
+        b->base[j] = jm != 0.0 ? c02 / (c02 + b->K2[j]) : 0.0;
+    }
+    for (int u = 0; u < b->nbu; ++u) b->total[u] = c->buffer_total[u];
+    for (int k = 0; k < b->nbb; ++k) {
+        b->bion[k] = c->bound_ion[k];
+        b->bbuf[k] = c->bound_buffer[k];
+        b->D[ni + k] = c->bound_D[k];
+        b->kon[k] = c->bound_kon[k];
+        b->koff[k] = c->bound_koff[k];
+    }
+    /* exchanged fields: every ion, then the dyad's mobile bound fields */
+    for (int j = 0; j < ni; ++j) {
+        b->ex_box[b->ne] = j;
+        b->ex_dyad[b->ne] = j;
+        b->ex_D[b->ne] = o->ion_D[j];
+        ++b->ne;
+    }
+    for (int db = 0; db < nB; ++db) {
+        int k = -1;
+        for (int t = 0; t < b->nbb; ++t)
+            if (c->bound_dyad[t] == db) k = t;
+        int face = o->buf_faces && o->buf_faces[o->bound_buf[db]] && o->dim == 3;
+        if (face || o->bound_D[db] == 0.0) continue;
+        if (k < 0) {
+            box_destroy(b);
+            return -1;
+        }
+        b->ex_box[b->ne] = ni + k;
+        b->ex_dyad[b->ne] = ni + db;
+        b->ex_D[b->ne] = o->bound_D[db];
+        ++b->ne;
+    }
+    /* rest state */
+    b->init = (double *)xcalloc(b->nf, sizeof(double));
+    for (int j = 0; j < ni; ++j) b->init[j] = o->ion_c0[j];
+    if (o->start_eq)
+        for (int u = 0; u < b->nbu; ++u) {
+            double kon[16], koff[16], cc[16], sol[16];
+            int idx[16], m = 0;
+            for (int k = 0; k < b->nbb; ++k)
+                if (b->bbuf[k] == u) {
+                    kon[m] = b->kon[k];
+                    koff[m] = b->koff[k];
+                    cc[m] = o->ion_c0[b->bion[k]];
+                    idx[m++] = k;
+                }
+            if (!m) continue;
+            oracle_buffer_equilibrium(m, kon, koff, cc, b->total[u], sol);
+            for (int t = 0; t < m; ++t) b->init[ni + idx[t]] = sol[t];
+        }
+    /* rim columns and links */
+    int nx = o->nx, ny = o->ny;
+    b->dnz = o->dim == 3 ? o->nz : 1;
+    b->ncol = 2 * nx + 2 * (ny - 2);
+    b->nlink = 2 * (nx - 2) + 2 * (ny - 2);
+    b->col_x = (int *)xcalloc(b->ncol, sizeof(int));
+    b->col_y = (int *)xcalloc(b->ncol, sizeof(int));
+    b->col_q = (int *)xcalloc(4 * b->ncol, sizeof(int));
+    b->col_w = (double *)xcalloc(4 * b->ncol, sizeof(double));
+    int bad = 0;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            if (!(x == 0 || x == nx - 1 || y == 0 || y == ny - 1)) continue;
+            int k = rim_col(o, x, y);
+            b->col_x[k] = x;
+            b->col_y[k] = y;
+            if (box_bilinear(b, c->origin_x + x * o->dx, c->origin_y + y * o->dy, b->col_q + 4 * k, b->col_w + 4 * k))
+                bad = 1;
+        }
+    if (bad) {
+        box_destroy(b);
+        return -1;
+    }
+    b->lk_rx = (int *)xcalloc(b->nlink, sizeof(int));
+    b->lk_ry = (int *)xcalloc(b->nlink, sizeof(int));
+    b->lk_ix = (int *)xcalloc(b->nlink, sizeof(int));
+    b->lk_iy = (int *)xcalloc(b->nlink, sizeof(int));
+    b->lk_col = (int *)xcalloc(b->nlink, sizeof(int));
+    b->lk_fac = (double *)xcalloc(b->nlink, sizeof(double));
+    double thick = o->dim == 3 ? o->dz : o->zext;
+    int l = 0;
+    for (int side = 0; side < 2; ++side) /* x-rims: interior neighbour along x */
+        for (int y = 1; y < ny - 1; ++y, ++l) {
+            b->lk_rx[l] = side ? nx - 1 : 0;
+            b->lk_ix[l] = side ? nx - 2 : 1;
+            b->lk_ry[l] = b->lk_iy[l] = y;
+            b->lk_fac[l] = o->dy * thick / o->dx;
+        }
+    for (int side = 0; side < 2; ++side) /* y-rims: interior neighbour along y */
+        for (int x = 1; x < nx - 1; ++x, ++l) {
+            b->lk_ry[l] = side ? ny - 1 : 0;
+            b->lk_iy[l] = side ? ny - 2 : 1;
+            b->lk_rx[l] = b->lk_ix[l] = x;
+            b->lk_fac[l] = o->dx * thick / o->dy;
+        }
+    for (l = 0; l < b->nlink; ++l) b->lk_col[l] = rim_col(o, b->lk_rx[l], b->lk_ry[l]);
+    b->zw = (double *)xcalloc(b->dnz, sizeof(double));
+    for (int z = 0; z < b->dnz; ++z) b->zw[z] = (o->dim == 3 && (z == 0 || z == b->dnz - 1)) ? 0.5 : 1.0;
+    /* deposit rows: each z = 0 node gathers its (link, weight) pairs in link order */
+    int nfn = b->nx * b->ny;
+    int *cnt = (int *)xcalloc(nfn, sizeof(int));
+    for (l = 0; l < b->nlink; ++l)
+        for (int q = 0; q < 4; ++q)
+            if (b->col_w[4 * b->lk_col[l] + q] != 0.0) ++cnt[b->col_q[4 * b->lk_col[l] + q]];
+    b->dep_row = (int *)xcalloc(nfn, sizeof(int));
+    int rows = 0, ent = 0;
+    for (int n = 0; n < nfn; ++n) {
+        b->dep_row[n] = cnt[n] ? rows++ : -1;
+        ent += cnt[n];
+    }
+    b->dep_ptr = (int *)xcalloc(rows + 1, sizeof(int));
+    b->dep_link = (int *)xcalloc(ent, sizeof(int));
+    b->dep_w = (double *)xcalloc(ent, sizeof(double));
+    for (int n = 0, r = 0; n < nfn; ++n)
+        if (cnt[n]) {
+            b->dep_ptr[r + 1] = b->dep_ptr[r] + cnt[n];
+            ++r;
+        }
+    int *fill = (int *)xcalloc(rows + 1, sizeof(int));
+    for (l = 0; l < b->nlink; ++l)
+        for (int q = 0; q < 4; ++q) {
+            int k = b->lk_col[l];
+            if (b->col_w[4 * k + q] == 0.0) continue;
+            int r = b->dep_row[b->col_q[4 * k + q]];
+            int at = b->dep_ptr[r] + fill[r]++;
+            b->dep_link[at] = l;
+            b->dep_w[at] = b->col_w[4 * k + q];
+        }
+    free(cnt);
+    free(fill);
+    long per = (long)S * b->NB;
+    b->u = (double *)xcalloc((size_t)b->nf * per, sizeof(double));
+    b->un = (double *)xcalloc((size_t)b->nf * per, sizeof(double));
+    b->lrate = (double *)xcalloc((size_t)b->ne * S * b->nlink, sizeof(double));
+    b->acc = (double *)xcalloc((size_t)b->ne * S * b->nlink, sizeof(double));
+    box_reset(b);
+    /* the dyad takes its rim nodes from the box from now on */
+    o->ncol = b->ncol;
+    o->rim_nodes = (double *)xcalloc((size_t)(ni + nB) * S * b->ncol, sizeof(double));
+    o->rim_mask = (int *)xcalloc(ni + nB, sizeof(int));
+    for (int e = 0; e < b->ne; ++e) o->rim_mask[b->ex_dyad[e]] = 1;
+    h->box = b;
+    return 0;
+}
+
+/* Rim values of every exchanged dyad field from the box's z = 0 face. */
+static void box_to_rim(OracleHybrid *h) {
+    OracleBox *b = h->box;
+    OracleDyad *o = h->o;
+    long per = (long)b->S * b->NB;
+    for (int e = 0; e < b->ne; ++e)
+        for (int s = 0; s < b->S; ++s) {
+            const double *u = b->u + b->ex_box[e] * per + (long)s * b->NB;
+            for (int k = 0; k < b->ncol; ++k) {
+                const int *q = b->col_q + 4 * k;
+                const double *w = b->col_w + 4 * k;
+                o->rim_nodes[((long)b->ex_dyad[e] * b->S + s) * b->ncol + k] =
+                    w[0] * u[q[0]] + w[1] * u[q[1]] + w[2] * u[q[2]] + w[3] * u[q[3]];
+            }
+        }
+}
+
+/* Rim-link effluxes D (u_in - u_rim) A / d of the dyad's current state. */
+static void box_link_rates(OracleHybrid *h) {
+    OracleBox *b = h->box;
+    OracleDyad *o = h->o;
+    for (int e = 0; e < b->ne; ++e) {
+        int f = b->ex_dyad[e];
+        const double *u = f < o->n_ions ? o->ions + (long)f * o->NE : o->buffers + (long)(f - o->n_ions) * o->NE;
+        for (int s = 0; s < b->S; ++s)
+            for (int l = 0; l < b->nlink; ++l) {
+                double t = 0.0;
+                for (int z = 0; z < b->dnz; ++z)
+                    t += b->zw[z] * (u[idx3(o, b->lk_ix[l], b->lk_iy[l], z, s)] - u[idx3(o, b->lk_rx[l], b->lk_ry[l], z, s)]);
+                b->lrate[((long)e * b->S + s) * b->nlink + l] = b->ex_D[e] * (b->lk_fac[l] * t);
+            }
+    }
+}
+
+/* One explicit box step of site s. */
+static void box_step_site(OracleBox *b, double dt, int s) {
+    long per = (long)b->S * b->NB, base = (long)s * b->NB;
+    int sy = b->nx, sz = b->nx * b->ny;
+    for (int n = 0; n < b->NB; ++n) {
+        int x = n % b->nx, t = n / b->nx, y = t % b->ny, z = t / b->ny;
+        if (b->far && (x == 0 || x == b->nx - 1 || y == 0 || y == b->ny - 1 || z == b->nz - 1)) {
+            for (int f = 0; f < b->nf; ++f) b->un[f * per + base + n] = b->u[f * per + base + n];
+            continue;
+        }
+        int xl = x > 0 ? n - 1 : n + 1, xr = x < b->nx - 1 ? n + 1 : n - 1;
+        int yl = y > 0 ? n - sy : n + sy, yr = y < b->ny - 1 ? n + sy : n - sy;
+        int zl = z > 0 ? n - sz : n + sz, zr = z < b->nz - 1 ? n + sz : n - sz;
+        double v[16], evo[16], fr[8], dep[16];
+        for (int f = 0; f < b->nf; ++f) {
+            const double *u = b->u + f * per + base;
+            double vc = u[n];
+            v[f] = vc;
+            double lap = (u[xr] - 2 * vc + u[xl]) * b->ih2 + (u[yr] - 2 * vc + u[yl]) * b->ih2 +
+                         (u[zr] - 2 * vc + u[zl]) * b->ih2;
+            evo[f] = b->D[f] * lap;
+        }
+        for (int u = 0; u < b->nbu; ++u) fr[u] = b->total[u];
+        for (int k = 0; k < b->nbb; ++k) fr[b->bbuf[k]] -= v[b->ni + k];
+        for (int j = 0; j < b->ni; ++j) {
+            double R = 0.0;
+            for (int k = 0; k < b->nbb; ++k) {
+                if (b->bion[k] != j) continue;
+                double R1 = b->koff[k] * v[b->ni + k] - b->kon[k] * v[j] * fr[b->bbuf[k]];
+                evo[b->ni + k] += -R1;
+                R += R1;
+            }
+            evo[j] += R;
+            if (b->jmax[j] != 0.0) {
+                double c2 = v[j] * v[j];
+                evo[j] -= b->jmax[j] * (c2 / (c2 + b->K2[j]) - b->base[j]);
+            }
+        }
+        for (int f = 0; f < b->nf; ++f) dep[f] = 0.0;
+        if (z == 0 && b->dep_row[n] >= 0) {
+            int r = b->dep_row[n];
+            for (int e = 0; e < b->ne; ++e) {
+                const double *acc = b->acc + ((long)e * b->S + s) * b->nlink;
+                double sum = 0.0;
+                for (int k = b->dep_ptr[r]; k < b->dep_ptr[r + 1]; ++k) sum += b->dep_w[k] * acc[b->dep_link[k]];
+                dep[b->ex_box[e]] = sum;
+            }
+        }
+        for (int f = 0; f < b->nf; ++f) b->un[f * per + base + n] = v[f] + evo[f] * dt + dep[f] * b->ifv;
+    }
+}
+
+/* After each dyad step: accumulate dt times the rates of the state the step
+ * started from; every M steps, one box step (all sites). */
+static void box_after_step(OracleHybrid *h, double dt) {
+    OracleBox *b = h->box;
+    long n = (long)b->ne * b->S * b->nlink;
+    for (long i = 0; i < n; ++i) b->acc[i] = b->count == 0 ? dt * b->lrate[i] : b->acc[i] + dt * b->lrate[i];
+    if (++b->count < b->M) return;
+    b->count = 0;
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < b->S; ++s) box_step_site(b, b->M * dt, s);
+    double *t = b->u;
+    b->u = b->un;
+    b->un = t;
+}
+
+int oracle_hybrid_box_fields(const OracleHybrid *h) { return h->box ? h->box->nf : 0; }
+
+/* One box field, [site][z][y][x]. */
+void oracle_hybrid_get_box(const OracleHybrid *h, int f, double *out) {
+    long per = (long)h->box->S * h->box->NB;
+    memcpy(out, h->box->u + f * per, sizeof(double) * per);
+}
+
+void oracle_hybrid_set_box(OracleHybrid *h, int f, const double *in) {
+    long per = (long)h->box->S * h->box->NB;
+    memcpy(h->box->u + f * per, in, sizeof(double) * per);
+}

@@ -92,6 +92,14 @@ void oracle_hybrid_get_jsr(const OracleHybrid *h, double *out);
 void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *bound);
 int oracle_hybrid_cyto_bound(const OracleHybrid *h);
 
+/* Extension: the 3D cytosol box of CasimCytosolBoxConfig (casim_dyad.h),
+ * the same coupling order as casim_dyad_step.  Returns 0, or -1 for a
+ * configuration it refuses.  Box fields are [site][z][y][x]. */
+int oracle_hybrid_attach_box(OracleHybrid *h, const CasimCytosolBoxConfig *c);
+int oracle_hybrid_box_fields(const OracleHybrid *h);
+void oracle_hybrid_get_box(const OracleHybrid *h, int f, double *out);
+void oracle_hybrid_set_box(OracleHybrid *h, int f, const double *in);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,10 @@ def load():
             "oracle_hybrid_get_jsr": (None, [P, _dp]),
             "oracle_hybrid_get_cytosol": (None, [P, _dp, _dp]),
             "oracle_hybrid_cyto_bound": (C.c_int, [P]),
+            "oracle_hybrid_attach_box": (C.c_int, [P, C.c_void_p]),
+            "oracle_hybrid_box_fields": (C.c_int, [P]),
+            "oracle_hybrid_get_box": (None, [P, C.c_int, _dp]),
+            "oracle_hybrid_set_box": (None, [P, C.c_int, _dp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(lib, n)
@@ -267,6 +271,24 @@ class OracleHybridRun:
         out = np.empty(self.spec.n_ions * self.spec.n_sites)
         self.lib.oracle_hybrid_get_jsr(self.h, d(out))
         return out.reshape(self.spec.n_ions, self.spec.n_sites)
+
+    def attach_box(self, box):
+        """The 3D cytosol box (config.CytosolBoxSpec) around every site."""
+        self._bc, self._bkeep = box.to_struct(self.spec)
+        self._box = box
+        if self.lib.oracle_hybrid_attach_box(self.h, C.byref(self._bc)) != 0:
+            raise ValueError("cytosol box refused")
+
+    def box_field(self, f):
+        """Box field f as [site, z, y, x]."""
+        b = self._box
+        out = np.empty(self.spec.n_sites * b.nz * b.ny * b.nx)
+        self.lib.oracle_hybrid_get_box(self.h, f, d(out))
+        return out.reshape(self.spec.n_sites, b.nz, b.ny, b.nx)
+
+    def set_box_field(self, f, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self.lib.oracle_hybrid_set_box(self.h, f, d(a))
 
     def cytosol(self):
         nb = self.lib.oracle_hybrid_cyto_bound(self.h)
