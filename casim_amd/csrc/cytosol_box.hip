@@ -122,15 +122,27 @@ __global__ void k_box_efflux(const T *__restrict__ g, BoxArgs a, double dt, int 
     }
 }
 
-// One explicit box step.  Block (64, 4): lanes along x, four y-rows; grid
-// (y-blocks, z, site).  Ions and bound fields sit in separate arrays whose
-// loops run to the compile-time bounds NI, NB with guards, so every array
-// index is a constant and the per-node values stay in registers.
+// One explicit box step.  Block (64, 4): lanes along x, four y-rows.  The
+// 1D grid is a multiple of 8 blocks; block b runs work unit
+// (b % 8) * (grid / 8) + b / 8, so each XCD (blocks are dealt to the 8 XCDs
+// in turn) takes a contiguous range of units -- whole sites, plane after
+// plane in z -- and finds the planes z-1 and z+1 in its own L2.  Ions and
+// bound fields sit in separate arrays whose loops run to the compile-time
+// bounds NI, NB with guards, so every array index is a constant and the
+// per-node values stay in registers.
 template <int NI, int NB>
 __global__ __launch_bounds__(256) void k_box_step(const double *__restrict__ bin, double *__restrict__ bout, BoxArgs a,
                                                   double dt) {
 #pragma clang fp contract(off)
-    const int x = threadIdx.x, y = blockIdx.x * 4 + threadIdx.y, z = blockIdx.y, s = blockIdx.z;
+    const int per_xcd = gridDim.x >> 3;
+    const int unit = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int nyb = (a.ny + 3) >> 2;
+    const int plane_units = nyb * a.nz;
+    const int s = unit / plane_units;
+    if (s >= a.S) return;
+    const int r = unit - s * plane_units;
+    const int z = r / nyb;
+    const int x = threadIdx.x, y = (r - z * nyb) * 4 + threadIdx.y;
     if (x >= a.nx || y >= a.ny) return;
     const int n = x + a.nx * (y + a.ny * z);
     const size_t per = (size_t)a.S * a.NB;
@@ -244,7 +256,9 @@ void box_efflux(CasimDyad *d, bool first, double dt) {
 
 void box_step(CasimDyad *d, double dt) {
     CytoBox *b = d->box;
-    dim3 grid((b->a.ny + 3) / 4, b->a.nz, d->P.S), block(64, 4);
+    const long units = (long)((b->a.ny + 3) / 4) * b->a.nz * d->P.S;
+    REQUIRE(units < (1L << 31) - 8, CASIM_E_INVALID, "cytosol box batch too large for one launch");
+    dim3 grid((unsigned)((units + 7) / 8 * 8)), block(64, 4);
     if (b->a.ni == 1 && b->a.nbb <= 3)
         hipLaunchKernelGGL((k_box_step<1, 3>), grid, block, 0, d->stream, b->b[b->cur], b->b[b->cur ^ 1], b->a, dt);
     else
