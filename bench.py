@@ -138,19 +138,34 @@ def surrogate_leg(workloads, n_sites, t_max, plain_value):
             "target": meta.get("target")}
 
 
-def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100):
+def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100, tuning=None):
     """The coupled configuration: 2D clefts of 201 x 201 nodes at h = 2 nm,
     each inside a 51^3 cytosol box at h = 20 nm (Ca2+, A, B, troponin; box
     step every 50 dyad steps), a depleting jSR per site, 64 RyRs, dt = 2 ns,
     FP64.  One step = channels + dyad sweep (+ box coupling)."""
+    from casim_amd import autotune
     sp = workloads.coupled_dyads(n_sites)
     box = workloads.coupled_box(sp)
+    dt = sp.meta["dt_ms"]
+    tuned = autotune.lookup(sp)
+    if tuning:
+        tuned = {"planes": tuning[0], "threads": tuning[1], "prefetch": tuning[2] if len(tuning) > 2 else 1,
+                 "sweep_ms": None}
+    if tuned is None:
+        # the sweep's launch shape, tuned on the same sites and pool without the box
+        probe = DyadHIP(sp)
+        probe.AttachPools(workloads.coupled_pools())
+        probe.InitOpeningAll()
+        probe.UpdatePools(*sp.rim_arrays())
+        tuned, _ = autotune.tune(probe, sp, dt, log=log)
+        probe.close()
+    log("coupled tuning:", {k: tuned[k] for k in ("planes", "threads", "prefetch", "sweep_ms")})
     dev = DyadHIP(sp)
     dev.AttachPools(workloads.coupled_pools())
     dev.AttachCytosolBox(box)
+    dev.SetTuning(tuned["planes"], tuned["threads"], tuned.get("prefetch", 1))
     dev.InitOpeningAll()
     dev.UpdatePools()
-    dt = sp.meta["dt_ms"]
     chunk = max(box.substeps, chunk // box.substeps * box.substeps)
     steps = max(chunk, steps // chunk * chunk)
     done = 0
@@ -168,7 +183,9 @@ def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100):
     ca = dev.GetCytosolBox(0)
     res = {"workload": sp.meta["workload"], "sites": n_sites, "dyad_grid": "%dx%d" % (sp.nx, sp.ny),
            "box_grid": "%dx%dx%d" % (box.nx, box.ny, box.nz), "box_substeps": box.substeps, "dt_ns": dt * 1e6,
-           "steps": steps, "wall_s": t, "ms_per_step": 1e3 * t / steps,
+           "steps": steps, "steps_per_pass": dev.StepsPerPass(),
+           "tuning": {k: tuned[k] for k in ("planes", "threads", "prefetch")},
+           "wall_s": t, "ms_per_step": 1e3 * t / steps,
            "value": n_sites * steps * dt / t, "unit": "dyad-ms/s",
            "sweep_kernel_ms": ms_sweep / chunk, "channel_kernel_ms": ms_chan / chunk,
            "sweep_gbs": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9,
@@ -219,11 +236,12 @@ def main():
     from casim_amd import abi, workloads
     from casim_amd.dyad import DyadHIP
     abi.load()
+    tuning = [int(v) for v in a.tuning.split(",")] if a.tuning else None
     if a.coupled_only:
-        print(json.dumps(coupled_leg(workloads, DyadHIP, a.coupled_sites, a.coupled_steps, a.warmup)), flush=True)
+        print(json.dumps(coupled_leg(workloads, DyadHIP, a.coupled_sites, a.coupled_steps, a.warmup,
+                                     tuning=tuning)), flush=True)
         return
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
-    tuning = [int(v) for v in a.tuning.split(",")] if a.tuning else None
     r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning,
                       a.steps_per_pass)
     log("fp64:", json.dumps(r64))

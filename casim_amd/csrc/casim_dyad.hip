@@ -128,7 +128,9 @@ void CasimDyad::launch_step(double dt) {
 
 bool CasimDyad::pair_ok() const {
     // a cytosol pool moves the Dirichlet rim every step: one step per pass
-    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || (Q.on && Q.cyto) || box) return false;
+    if (steps_per_pass < 2 || !use_sweep || sweep_threads <= 0 || (Q.on && Q.cyto)) return false;
+    // a cytosol box steps between passes only with an even number of substeps
+    if (box && casim::box_substeps(this) % 2 != 0) return false;
     return 4 * (size_t)P.n_grid * P.ny * P.nz * tsize <= 64 * 1024;
 }
 
@@ -792,6 +794,7 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                         // steps k and k+1 in one pass, then step k+1's channel
                         // update from the mouth values the pass stored
                         d->launch_pair(dt);
+                        if (d->box) casim::box_efflux_ring(d, dt);  // the middle step's rim effluxes
                         d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                            dt);
                         k += 2;
@@ -862,6 +865,7 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                 if (pairs && k + 1 < k_steps) {
                     d->launch_pair(dt);
                     HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
+                    if (d->box) casim::box_efflux_ring(d, dt);
                     HIPCK(hipEventRecord(ev[4 * (k + 1)], d->stream));
                     d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                            dt);

@@ -56,6 +56,8 @@ struct BoxArgs {
     const int *link_rim;    // [nlink] dyad flat node of the rim end (z = 0)
     const int *link_in;     // [nlink] dyad flat node of the interior end (z = 0)
     const int *link_col;    // [nlink] rim column
+    const int *link_ring;   // [nlink] ring index of the interior end (ring_index)
+    int RN;                 // ring values per site and grid field (ring nodes x dyad z nodes)
     const double *link_fac; // [nlink] A / d per unit z weight
     const double *zw;       // [dnz] z weights (1 in 2D; 1/2 on the two z-planes in 3D)
     const int *dep_row;     // [nx*ny] deposit row of a z = 0 node, or -1
@@ -70,6 +72,7 @@ struct CytoBox {
     int substeps = 1;
     int cur = 0;
     double *b[2] = {nullptr, nullptr};
+    void *ring = nullptr;      // [grid field][site][RN] step-k+1 values written by k_sweep2
     std::vector<double> init;  // [nf] rest values
     std::vector<void *> owned;
 };
@@ -103,19 +106,23 @@ __global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs 
 }
 
 // Rim-link effluxes of the dyad state g, times dt, into the accumulator:
-// one thread per (link, site).
+// one thread per (link, site).  With ring non-null the interior ends are read
+// from the ring values of the middle step of a two-step pass (the rim is the
+// same for both steps).
 template <typename T>
-__global__ void k_box_efflux(const T *__restrict__ g, BoxArgs a, double dt, int first) {
+__global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring, BoxArgs a, double dt, int first) {
 #pragma clang fp contract(off)
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (l >= a.nlink) return;
     const size_t rim = (size_t)s * a.dNS + a.link_rim[l], in = (size_t)s * a.dNS + a.link_in[l];
+    const size_t rin = (size_t)s * a.RN + (size_t)a.link_ring[l] * a.dnz;
     const double fac = a.link_fac[l];
     for (int e = 0; e < a.ne; ++e) {
         const T *u = g + a.ex_grid[e] * a.dfs;
+        const T *ui = ring ? ring + (size_t)a.ex_grid[e] * a.S * a.RN + rin : u + in;
         double t = 0.0;
-        for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)u[in + z] - (double)u[rim + z]);
+        for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)ui[z] - (double)u[rim + z]);
         const double ev = a.ex_D[e] * (fac * t);
         double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink + l;
         *acc = first ? dt * ev : *acc + dt * ev;
@@ -242,17 +249,23 @@ T *upload(CytoBox *b, const std::vector<T> &h, hipStream_t st) {
 
 }  // namespace
 
-void box_efflux(CasimDyad *d, bool first, double dt) {
+namespace {
+void efflux_launch(CasimDyad *d, const void *ring, bool first, double dt) {
     CytoBox *b = d->box;
     dim3 grid((b->a.nlink + 127) / 128, d->P.S);
     if (d->precision == CASIM_FP64)
-        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)d->g[d->cur], b->a,
-                           dt, (int)first);
+        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)d->g[d->cur],
+                           (const double *)ring, b->a, dt, (int)first);
     else
-        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)d->g[d->cur], b->a, dt,
-                           (int)first);
+        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)d->g[d->cur],
+                           (const float *)ring, b->a, dt, (int)first);
     HIPCK(hipGetLastError());
 }
+}  // namespace
+
+void box_efflux(CasimDyad *d, bool first, double dt) { efflux_launch(d, nullptr, first, dt); }
+void box_efflux_ring(CasimDyad *d, double dt) { efflux_launch(d, d->box->ring, false, dt); }
+void *box_ring(const CasimDyad *d) { return d->box ? d->box->ring : nullptr; }
 
 void box_step(CasimDyad *d, double dt) {
     CytoBox *b = d->box;
@@ -472,7 +485,7 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
                 col_node[k] = (x * ny + y) * nzd;
                 bilinear(a, c->h, c->origin_x + x * d->dx, c->origin_y + y * d->dy, &col_box[4 * k], &col_w[4 * k]);
             }
-            std::vector<int> link_rim(a.nlink), link_in(a.nlink), link_col(a.nlink);
+            std::vector<int> link_rim(a.nlink), link_in(a.nlink), link_col(a.nlink), link_ring(a.nlink);
             std::vector<double> link_fac(a.nlink);
             const double thick = P.dim == 3 ? d->dz : d->zext;
             for (int l = 0; l < a.nlink; ++l) {
@@ -496,6 +509,7 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
                 }
                 link_rim[l] = (x * ny + y) * nzd;
                 link_in[l] = (xi * ny + yi) * nzd;
+                link_ring[l] = ring_index(xi, yi, nx, ny);
                 link_col[l] = colof(x, y);
                 link_fac[l] = fac;
             }
@@ -525,6 +539,10 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
             a.link_rim = upload(b, link_rim, d->stream);
             a.link_in = upload(b, link_in, d->stream);
             a.link_col = upload(b, link_col, d->stream);
+            a.link_ring = upload(b, link_ring, d->stream);
+            a.RN = ring_nodes(nx, ny) * a.dnz;
+            HIPCK(hipMalloc(&b->ring, d->tsize * (size_t)P.n_grid * P.S * a.RN));
+            b->owned.push_back(b->ring);
             a.link_fac = upload(b, link_fac, d->stream);
             a.zw = upload(b, zw, d->stream);
             a.dep_row = upload(b, dep_row, d->stream);

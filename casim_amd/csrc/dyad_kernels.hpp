@@ -549,6 +549,21 @@ struct PlaneSlots {
     }
 };
 
+// Index of interior node (x, y) in the ring of nodes next to the rim (x = 1,
+// x = nx-2, y = 1, y = ny-2), or -1: x = 1 first, then x = nx-2, then the
+// y = 1 and y = ny-2 rows without their ends.  The cytosol box reads the
+// step-k+1 values that k_sweep2 leaves there.
+__host__ __device__ __forceinline__ int ring_index(int x, int y, int nx, int ny) {
+    if (x == 1) return y - 1;
+    if (x == nx - 2) return (ny - 2) + y - 1;
+    if (y == 1) return 2 * (ny - 2) + x - 2;
+    if (y == ny - 2) return 2 * (ny - 2) + (nx - 4) + x - 2;
+    return -1;
+}
+__host__ __device__ __forceinline__ int ring_nodes(int nx, int ny) {
+    return 2 * (ny - 2) + 2 * (nx > 4 ? nx - 4 : 0);
+}
+
 // First entry of a site's node-sorted channel list at or after plane x.
 __device__ __forceinline__ int first_channel_at(const int *__restrict__ snode, int lo, int hi, int key) {
     while (lo < hi) {
@@ -713,14 +728,14 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 // the pool's value, which the channel call before the pass has advanced.  Chunks of one site recompute one halo plane of
 // step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
 // two steps here equal two k_sweep steps bit for bit.
-template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL>
+template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL, bool RING>
 __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
                                                 const T *__restrict__ fin, T *__restrict__ fout,
                                                 const int *__restrict__ ch_off, const int *__restrict__ snode,
                                                 const int *__restrict__ sperm, const double *__restrict__ sJ,
                                                 const int *__restrict__ cstate, const int *__restrict__ ctype,
                                                 const double *__restrict__ jsr, const double *__restrict__ jnew,
-                                                double *__restrict__ mouths,
+                                                double *__restrict__ mouths, T *__restrict__ ring,
                                                 const Channels a, Params p, double dt_, int chunk, int nchunks,
                                                 int units) {
     constexpr int LI = NI ? NI : MAXI;
@@ -834,6 +849,22 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                     for (int g = 0; g < NG; ++g) A[2][k][g] = R[1][k][g];
 #pragma unroll
                     for (int f = 0; f < NF; ++f) FA[1][k][f] = F[0][k][f];
+                }
+            }
+            // step-k+1 values of the nodes next to the rim, for a cytosol box
+            if (RING && interiorA) {
+                const int nzd = DIM == 3 ? p.nz : 1;
+                const size_t RN = (size_t)ring_nodes(p.nx, p.ny) * nzd;
+#pragma unroll
+                for (int k = 0; k < SL; ++k) {
+                    if (!ps.inner[k]) continue;
+                    const int ri = ring_index(x, ps.yy[k], p.nx, p.ny);
+                    if (ri < 0) continue;
+                    const int z = DIM == 3 ? ps.q[k] % p.nz : 0;
+                    T *rp = ring + (size_t)s * RN + (size_t)ri * nzd + z;
+#pragma unroll
+                    for (int g = 0; g < NG; ++g)
+                        if (g < ng) rp[g * (size_t)p.S * RN] = A[2][k][g];
                 }
             }
         }

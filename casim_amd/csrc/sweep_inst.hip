@@ -57,8 +57,8 @@ int layout_id(const Params &p) {
     return 0;
 }
 
-template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int SL>
-void sweep2_launch(CasimDyad *d, double dt) {
+template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int SL, bool RING>
+void sweep2_ring(CasimDyad *d, double dt) {
     const Params &p = d->P;
     const int P = p.ny * p.nz;
     const int chunk = d->sweep_chunk;
@@ -66,11 +66,20 @@ void sweep2_launch(CasimDyad *d, double dt) {
     const int units = p.S * nchunks;
     const int grid = (units + 7) / 8 * 8;
     const size_t lds = 4 * (size_t)p.n_grid * P * sizeof(T);
-    hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL>), dim3(grid), dim3(d->sweep_threads), lds, d->stream,
+    hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL, RING>), dim3(grid), dim3(d->sweep_threads), lds,
+                       d->stream,
                        (const T *)d->g[d->cur], (T *)d->g[d->cur ^ 1], (const T *)d->f[d->cur],
                        (T *)d->f[d->cur ^ 1], d->d_off, d->d_snode, d->d_sperm, d->d_sJ, d->d_state, d->d_type,
-                       d->d_jsr, (const double *)(d->Q.on ? d->Q.jnew : nullptr), d->d_mouths, d->Ch, d->P, dt,
-                       chunk, nchunks, units);
+                       d->d_jsr, (const double *)(d->Q.on ? d->Q.jnew : nullptr), d->d_mouths,
+                       (T *)box_ring(d), d->Ch, d->P, dt, chunk, nchunks, units);
+}
+
+// The ring store (step-k+1 values next to the rim, for a cytosol box) is a
+// separate instantiation so that the plain path carries none of its code.
+template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int SL>
+void sweep2_launch(CasimDyad *d, double dt) {
+    if (box_ring(d)) return sweep2_ring<T, DIM, NI, NB, NG, NF, LY, SL, true>(d, dt);
+    return sweep2_ring<T, DIM, NI, NB, NG, NF, LY, SL, false>(d, dt);
 }
 
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY>

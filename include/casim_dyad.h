@@ -321,7 +321,8 @@ int casim_dyad_get_open_counts(CasimDyad *d, uint8_t *out);
 
 /* Steps per state pass of casim_dyad_step: 2 (the default) advances pairs of
  * steps in one read and one write of the state, where the configuration
- * allows it (plane-marching kernel, no cytosol pool); 1 forces one pass per
+ * allows it (plane-marching kernel, no cytosol pool, a cytosol box only with
+ * an even number of substeps); 1 forces one pass per
  * step.  Results do not depend on it.  casim_dyad_steps_per_pass returns the
  * value in effect (1 or 2), -1 for a null handle. */
 int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps);
@@ -378,9 +379,10 @@ typedef struct CasimCytosolBoxConfig {
 
 /* Attach a cytosol box to every site (not with a cytosol pool; a jSR pool is
  * fine).  Afterwards casim_dyad_update takes NULL rims (the box sets them),
- * casim_dyad_step needs k_steps to be a multiple of substeps and runs one
- * step per state pass, and casim_dyad_reset also resets the box.  Requires
- * the bilinear support of every rim node to lie off the box's x and y faces. */
+ * casim_dyad_step needs k_steps to be a multiple of substeps (two steps per
+ * state pass need an even substeps), casim_dyad_run_rd is refused, and
+ * casim_dyad_reset also resets the box.  Requires the bilinear support of
+ * every rim node to lie off the box's x and y faces and nx <= 64. */
 int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c);
 /* Box fields: the dyad's ions, then the box's bound fields; 0 without a box. */
 int32_t casim_dyad_cytosol_box_fields(const CasimDyad *d);

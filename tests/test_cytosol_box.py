@@ -273,23 +273,28 @@ def compare_all(ora, dev, sp, box, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dim,precision", [(2, abi.CASIM_FP64), (3, abi.CASIM_FP64), (2, abi.CASIM_FP32)])
-def test_gpu_box_matches_oracle(gpu, oracle_mod, dim, precision):
+@pytest.mark.parametrize("dim,precision,substeps", [(2, abi.CASIM_FP64, 5), (3, abi.CASIM_FP64, 5),
+                                                    (2, abi.CASIM_FP32, 5), (2, abi.CASIM_FP64, 4),
+                                                    (3, abi.CASIM_FP64, 4)])
+def test_gpu_box_matches_oracle(gpu, oracle_mod, dim, precision, substeps):
     """Gating channels (FP64; in FP32 the opened channel stays open, as a
     rounding-level difference can flip a draw), a depleting jSR, SERCA and
     held faces: dyad fields, box fields, channel histories and the jSR after
-    several calls of different lengths (graphs of both box-buffer parities)."""
-    say("box parity", dim, precision)
+    several calls of different lengths (graphs of both box-buffer parities).
+    Odd substeps run one step per pass, even substeps two."""
+    say("box parity", dim, precision, substeps)
     fp64 = precision == abi.CASIM_FP64
     sp, box = small_case(dim=dim, n_sites=3, clamped=not fp64, far=True, serca={"Calcium": (0.2, 0.184)},
-                         precision=precision)
+                         precision=precision, substeps=substeps)
     if fp64:
         sp.channel_types[0].params["kopen0"] *= 1e4
         sp.channel_types[0].params["kclose0"] *= 10
     ora, dev = gpu_pair(oracle_mod, sp, box, depleting_pool(), perturbed=True)
     tol = FP64_TOL if precision == abi.CASIM_FP64 else FP32_TOL
     dt = 2e-6
-    for k in (5, 10, 15, 5, 20):
+    assert dev.StepsPerPass() == (2 if substeps % 2 == 0 else 1)
+    for k in (1, 2, 3, 1, 4):
+        k *= substeps
         dev.Step(dt, k)
         ora.step(dt, k)
         if precision == abi.CASIM_FP64:
@@ -297,6 +302,43 @@ def test_gpu_box_matches_oracle(gpu, oracle_mod, dim, precision):
     compare_all(ora, dev, sp, box, tol)
     assert rel_err(dev.GetPools()[0], ora.jsr()) < tol
     assert (dev.GetChannelsStates() >= 0).any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,precision", [(2, abi.CASIM_FP64), (3, abi.CASIM_FP64), (2, abi.CASIM_FP32)])
+def test_gpu_box_two_step_passes_equal_single_steps(gpu, dim, precision):
+    """With a box of even substeps, pairs of steps per pass (the middle
+    step's rim effluxes from the ring values the pass writes) give the
+    one-step schedule's results bit for bit: dyad and box fields, channel
+    histories, RNG and jSR, with fast gating."""
+    from casim_amd.dyad import DyadHIP
+    say("box pairs", dim, precision)
+    sp, box = small_case(dim=dim, n_sites=3, clamped=False, far=True, serca={"Calcium": (0.2, 0.184)},
+                         precision=precision, substeps=4)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    sp.channel_types[0].params["kclose0"] *= 100
+    devs = []
+    for steps in (1, 2):
+        d = DyadHIP(sp)
+        d.AttachPools(depleting_pool())
+        d.AttachCytosolBox(box)
+        d.SetStepsPerPass(steps)
+        perturb(np.random.default_rng(7), sp, box, d.SetField, d.SetCytosolBox, d.GetCytosolBox)
+        d.InitOpeningAll()
+        d.UpdatePools()
+        for k in (4, 8, 12, 4):
+            d.Step(2e-6, k)
+        devs.append(d)
+    one, two = devs
+    assert one.StepsPerPass() == 1 and two.StepsPerPass() == 2
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(one.GetField(f), two.GetField(f)), f
+    for f in range(one.CytosolBoxFields()):
+        assert np.array_equal(one.GetCytosolBox(f), two.GetCytosolBox(f)), f
+    st = one.GetChannelsStates()
+    assert np.array_equal(st, two.GetChannelsStates())
+    assert np.array_equal(one.GetRngStates(), two.GetRngStates())
+    assert np.array_equal(one.GetPools()[0], two.GetPools()[0])
 
 
 @pytest.mark.gpu
