@@ -53,11 +53,11 @@ struct BoxArgs {
     const int *col_node;    // [ncol] dyad flat node of the column at z = 0
     const int *col_box;     // [ncol*4] box face nodes
     const double *col_w;    // [ncol*4] bilinear weights
-    const int *link_rim;    // [nlink] dyad flat node of the rim end (z = 0)
     const int *link_in;     // [nlink] dyad flat node of the interior end (z = 0)
     const int *link_col;    // [nlink] rim column
     const int *link_ring;   // [nlink] ring index of the interior end (ring_index)
     int RN;                 // ring values per site and grid field (ring nodes x dyad z nodes)
+    void *rimc;             // [ne][S][ncol] the rim values k_box_rim wrote (dyad precision)
     const double *link_fac; // [nlink] A / d per unit z weight
     const double *zw;       // [dnz] z weights (1 in 2D; 1/2 on the two z-planes in 3D)
     const int *dep_row;     // [nx*ny] deposit row of a z = 0 node, or -1
@@ -102,6 +102,7 @@ __global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs 
         const double v = w[0] * u[q[0]] + w[1] * u[q[1]] + w[2] * u[q[2]] + w[3] * u[q[3]];
         const size_t o = a.ex_grid[e] * a.dfs + node;
         for (int z = 0; z < a.dnz; ++z) g0[o + z] = g1[o + z] = (T)v;
+        ((T *)a.rimc)[((size_t)e * a.S + s) * a.ncol + c] = (T)v;
     }
 }
 
@@ -115,14 +116,16 @@ __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
     if (l >= a.nlink) return;
-    const size_t rim = (size_t)s * a.dNS + a.link_rim[l], in = (size_t)s * a.dNS + a.link_in[l];
+    const size_t in = (size_t)s * a.dNS + a.link_in[l];
     const size_t rin = (size_t)s * a.RN + (size_t)a.link_ring[l] * a.dnz;
     const double fac = a.link_fac[l];
+    const T *rimc = (const T *)a.rimc;
     for (int e = 0; e < a.ne; ++e) {
-        const T *u = g + a.ex_grid[e] * a.dfs;
-        const T *ui = ring ? ring + (size_t)a.ex_grid[e] * a.S * a.RN + rin : u + in;
+        const T *ui = ring ? ring + (size_t)a.ex_grid[e] * a.S * a.RN + rin : g + a.ex_grid[e] * a.dfs + in;
+        // the rim node's value (every z of its column): what k_box_rim wrote there
+        const double ur = (double)rimc[((size_t)e * a.S + s) * a.ncol + a.link_col[l]];
         double t = 0.0;
-        for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)ui[z] - (double)u[rim + z]);
+        for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)ui[z] - ur);
         const double ev = a.ex_D[e] * (fac * t);
         double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink + l;
         *acc = first ? dt * ev : *acc + dt * ev;
@@ -485,7 +488,7 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
                 col_node[k] = (x * ny + y) * nzd;
                 bilinear(a, c->h, c->origin_x + x * d->dx, c->origin_y + y * d->dy, &col_box[4 * k], &col_w[4 * k]);
             }
-            std::vector<int> link_rim(a.nlink), link_in(a.nlink), link_col(a.nlink), link_ring(a.nlink);
+            std::vector<int> link_in(a.nlink), link_col(a.nlink), link_ring(a.nlink);
             std::vector<double> link_fac(a.nlink);
             const double thick = P.dim == 3 ? d->dz : d->zext;
             for (int l = 0; l < a.nlink; ++l) {
@@ -507,7 +510,6 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
                     xi = x;
                     fac = d->dx * thick / d->dy;
                 }
-                link_rim[l] = (x * ny + y) * nzd;
                 link_in[l] = (xi * ny + yi) * nzd;
                 link_ring[l] = ring_index(xi, yi, nx, ny);
                 link_col[l] = colof(x, y);
@@ -536,13 +538,14 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
             a.col_node = upload(b, col_node, d->stream);
             a.col_box = upload(b, col_box, d->stream);
             a.col_w = upload(b, col_w, d->stream);
-            a.link_rim = upload(b, link_rim, d->stream);
             a.link_in = upload(b, link_in, d->stream);
             a.link_col = upload(b, link_col, d->stream);
             a.link_ring = upload(b, link_ring, d->stream);
             a.RN = ring_nodes(nx, ny) * a.dnz;
             HIPCK(hipMalloc(&b->ring, d->tsize * (size_t)P.n_grid * P.S * a.RN));
             b->owned.push_back(b->ring);
+            HIPCK(hipMalloc(&a.rimc, d->tsize * (size_t)a.ne * P.S * a.ncol));
+            b->owned.push_back(a.rimc);
             a.link_fac = upload(b, link_fac, d->stream);
             a.zw = upload(b, zw, d->stream);
             a.dep_row = upload(b, dep_row, d->stream);
