@@ -189,6 +189,7 @@ def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100, tuning=No
            "value": n_sites * steps * dt / t, "unit": "dyad-ms/s",
            "sweep_kernel_ms": ms_sweep / chunk, "channel_kernel_ms": ms_chan / chunk,
            "sweep_gbs": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9,
+           "sweep_frac_of_hbm_peak": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "jsr_min_uM": float(jsr.min()), "box_ca_max_uM": float(ca.max()),
            "finite": bool(np.isfinite(ca).all() and np.isfinite(dev.GetField(0)).all())}
     dev.close()

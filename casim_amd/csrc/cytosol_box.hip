@@ -74,6 +74,7 @@ struct CytoBox {
     double *b[2] = {nullptr, nullptr};
     void *ring = nullptr;      // [grid field][site][RN] step-k+1 values written by k_sweep2
     std::vector<double> init;  // [nf] rest values
+    double *d_init = nullptr;  // the same on the device
     std::vector<void *> owned;
 };
 
@@ -299,13 +300,10 @@ void box_rim(CasimDyad *d) {
 void box_reset(CasimDyad *d) {
     CytoBox *b = d->box;
     const size_t per = (size_t)b->a.S * b->a.NB;
-    double *init = upload(b, b->init, d->stream);
     hipLaunchKernelGGL(k_box_init, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, d->stream, b->b[0], b->b[1],
-                       init, b->a);
+                       b->d_init, b->a);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(d->stream));
-    HIPCK(hipFree(init));
-    b->owned.pop_back();
     b->cur = 0;
 }
 
@@ -558,6 +556,7 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
                 HIPCK(hipMalloc(&b->b[i], sizeof(double) * per));
                 b->owned.push_back(b->b[i]);
             }
+            b->d_init = upload(b, b->init, d->stream);
             b->substeps = c->substeps;
             d->box = b;
             box_reset(d);

@@ -386,7 +386,8 @@ typedef struct CasimCytosolBoxConfig {
 int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c);
 /* Box fields: the dyad's ions, then the box's bound fields; 0 without a box. */
 int32_t casim_dyad_cytosol_box_fields(const CasimDyad *d);
-/* One box field, [site][z][y][x] (x fastest). */
+/* One box field, [site][z][y][x] (x fastest).  A set takes effect on the
+ * dyad's rim at the next casim_dyad_update (call it before stepping). */
 int casim_dyad_get_cytosol_box(CasimDyad *d, int32_t field, double *out);
 int casim_dyad_set_cytosol_box(CasimDyad *d, int32_t field, const double *in);
 
