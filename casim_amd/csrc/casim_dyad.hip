@@ -784,17 +784,18 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                         d->launch_rim(d->Q.cions, d->Q.cbb);
                         d->launch_efflux();
                     }
-                    if (d->box) casim::box_efflux(d, k % M == 0, dt);
+                    const bool pair = pairs && k + 1 < k_steps;
+                    if (d->box && !pair) casim::box_efflux(d, k % M == 0, dt);
                     // step 1 recomputes the currents of the preceding update, so
                     // it does not add to the accumulated totals again
                     d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
                                            CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                        dt);
-                    if (pairs && k + 1 < k_steps) {
+                    if (pair) {
                         // steps k and k+1 in one pass, then step k+1's channel
                         // update from the mouth values the pass stored
                         d->launch_pair(dt);
-                        if (d->box) casim::box_efflux_ring(d, dt);  // the middle step's rim effluxes
+                        if (d->box) casim::box_efflux_pair(d, k % M == 0, dt);  // both steps' rim effluxes
                         d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                            dt);
                         k += 2;
@@ -855,17 +856,18 @@ int casim_dyad_step_profiled(CasimDyad *d, double dt, int32_t k_steps, double ke
                     d->launch_rim(d->Q.cions, d->Q.cbb);
                     d->launch_efflux();
                 }
-                if (d->box) casim::box_efflux(d, k % M == 0, dt);
+                const bool pair = pairs && k + 1 < k_steps;
+                if (d->box && !pair) casim::box_efflux(d, k % M == 0, dt);
                 HIPCK(hipEventRecord(ev[4 * k], d->stream));
                 d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) | CH_POOL_UPDATE |
                                        CH_POOL_ADVANCE,
                                    dt);
                 HIPCK(hipEventRecord(ev[4 * k + 1], d->stream));
                 HIPCK(hipEventRecord(ev[4 * k + 2], d->stream));
-                if (pairs && k + 1 < k_steps) {
+                if (pair) {
                     d->launch_pair(dt);
                     HIPCK(hipEventRecord(ev[4 * k + 3], d->stream));
-                    if (d->box) casim::box_efflux_ring(d, dt);
+                    if (d->box) casim::box_efflux_pair(d, k % M == 0, dt);
                     HIPCK(hipEventRecord(ev[4 * (k + 1)], d->stream));
                     d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
                                            dt);

@@ -108,9 +108,10 @@ __global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs 
 }
 
 // Rim-link effluxes of the dyad state g, times dt, into the accumulator:
-// one thread per (link, site).  With ring non-null the interior ends are read
-// from the ring values of the middle step of a two-step pass (the rim is the
-// same for both steps).
+// one thread per (link, site).  With ring non-null the thread then adds the
+// next step's effluxes too, from the ring values a two-step pass left (the
+// rim is the same for both steps): acc + dt e(k), then + dt e(k+1), the
+// order of two separate calls.
 template <typename T>
 __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring, BoxArgs a, double dt, int first) {
 #pragma clang fp contract(off)
@@ -122,14 +123,22 @@ __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring
     const double fac = a.link_fac[l];
     const T *rimc = (const T *)a.rimc;
     for (int e = 0; e < a.ne; ++e) {
-        const T *ui = ring ? ring + (size_t)a.ex_grid[e] * a.S * a.RN + rin : g + a.ex_grid[e] * a.dfs + in;
         // the rim node's value (every z of its column): what k_box_rim wrote there
         const double ur = (double)rimc[((size_t)e * a.S + s) * a.ncol + a.link_col[l]];
+        const T *ui = g + a.ex_grid[e] * a.dfs + in;
         double t = 0.0;
         for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)ui[z] - ur);
-        const double ev = a.ex_D[e] * (fac * t);
         double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink + l;
-        *acc = first ? dt * ev : *acc + dt * ev;
+        const double ev = a.ex_D[e] * (fac * t);
+        double v = first ? dt * ev : *acc + dt * ev;
+        if (ring) {
+            const T *ur1 = ring + (size_t)a.ex_grid[e] * a.S * a.RN + rin;
+            double t1 = 0.0;
+            for (int z = 0; z < a.dnz; ++z) t1 += a.zw[z] * ((double)ur1[z] - ur);
+            const double ev1 = a.ex_D[e] * (fac * t1);
+            v = v + dt * ev1;
+        }
+        *acc = v;
     }
 }
 
@@ -254,21 +263,25 @@ T *upload(CytoBox *b, const std::vector<T> &h, hipStream_t st) {
 }  // namespace
 
 namespace {
-void efflux_launch(CasimDyad *d, const void *ring, bool first, double dt) {
+void efflux_launch(CasimDyad *d, const void *g, const void *ring, bool first, double dt) {
     CytoBox *b = d->box;
     dim3 grid((b->a.nlink + 127) / 128, d->P.S);
     if (d->precision == CASIM_FP64)
-        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)d->g[d->cur],
+        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)g,
                            (const double *)ring, b->a, dt, (int)first);
     else
-        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)d->g[d->cur],
+        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)g,
                            (const float *)ring, b->a, dt, (int)first);
     HIPCK(hipGetLastError());
 }
 }  // namespace
 
-void box_efflux(CasimDyad *d, bool first, double dt) { efflux_launch(d, nullptr, first, dt); }
-void box_efflux_ring(CasimDyad *d, double dt) { efflux_launch(d, d->box->ring, false, dt); }
+void box_efflux(CasimDyad *d, bool first, double dt) { efflux_launch(d, d->g[d->cur], nullptr, first, dt); }
+// after a two-step pass: the pass's input buffer (now g[cur ^ 1]) still holds
+// step k, and the ring holds step k+1 next to the rim
+void box_efflux_pair(CasimDyad *d, bool first, double dt) {
+    efflux_launch(d, d->g[d->cur ^ 1], d->box->ring, first, dt);
+}
 void *box_ring(const CasimDyad *d) { return d->box ? d->box->ring : nullptr; }
 
 void box_step(CasimDyad *d, double dt) {

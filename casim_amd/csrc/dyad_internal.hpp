@@ -130,9 +130,10 @@ namespace casim {
 // the box's accumulator (first = overwrite); box_step: one box step of
 // substeps*dt consuming the accumulator; box_rim: dyad rim nodes from the box.
 void box_efflux(CasimDyad *d, bool first, double dt);
-// box_efflux_ring: the same for the middle step of a two-step pass, from the
-// ring values k_sweep2 wrote; box_ring: that buffer (null without a box).
-void box_efflux_ring(CasimDyad *d, double dt);
+// box_efflux_pair: after a two-step pass, both steps' effluxes in one launch
+// (the pass's input buffer for the first, the ring values k_sweep2 wrote for
+// the second); box_ring: that ring buffer (null without a box).
+void box_efflux_pair(CasimDyad *d, bool first, double dt);
 void *box_ring(const CasimDyad *d);
 void box_step(CasimDyad *d, double dt);
 void box_rim(CasimDyad *d);
