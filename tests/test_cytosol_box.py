@@ -360,6 +360,13 @@ def test_gpu_box_rejects_bad_calls(gpu):
     with pytest.raises(abi.CasimError):
         dev.GetCytosolBox(9)
     assert dev.StepsPerPass() == 1
+    # a box row must fit one wave (nx <= 64), and the rim's support must lie inside the face
+    for bad in (CytosolBoxSpec.centred(sp, 65, 9, 5, 10.0, 5, box.buffers),
+                CytosolBoxSpec(9, 9, 5, 10.0, (5.0, 20.0), 5, box.buffers)):
+        other = DyadHIP(sp)
+        with pytest.raises(abi.CasimError):
+            other.AttachCytosolBox(bad)
+        other.close()
 
 
 @pytest.mark.gpu
