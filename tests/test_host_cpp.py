@@ -26,7 +26,8 @@ def test_host_layer_builds_against_the_header():
 @pytest.mark.gpu
 def test_host_check_protocol_equals_fused(gpu):
     exe = os.path.join(HOST, "host_check")
-    assert os.path.exists(exe), "build first: make -C casim_amd/host"
+    if not os.path.exists(exe):  # a tree without build products: the host part builds in seconds
+        subprocess.run(["make", "-s", "-C", HOST], check=True, timeout=300)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     out = json.loads(r.stdout.strip().splitlines()[-1])
