@@ -177,24 +177,34 @@ __global__ __launch_bounds__(256) void k_box_step(const double *__restrict__ bin
     const int xl = x > 0 ? n - 1 : n + 1, xr = x < a.nx - 1 ? n + 1 : n - 1;
     const int yl = y > 0 ? n - sy : n + sy, yr = y < a.ny - 1 ? n + sy : n - sy;
     const int zl = z > 0 ? n - sz : n + sz, zr = z < a.nz - 1 ? n + sz : n - sz;
-    auto diffuse = [&](int f, double &vo, double &eo) {
-        const double *u = ib + f * per;
-        const double vc = u[n];
+    // all loads first (7 per field), so that every field's memory latency overlaps
+    constexpr int NT = NI + NB;
+    double q[NT][7];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const bool on = t < NI ? t < a.ni : t - NI < a.nbb;
+        const double *u = ib + (t < NI ? t : a.ni + t - NI) * per;
+        const int o[7] = {n, xl, xr, yl, yr, zl, zr};
+#pragma unroll
+        for (int k = 0; k < 7; ++k) q[t][k] = on ? u[o[k]] : 0.0;
+    }
+    auto diffuse = [&](int t, int f, double &vo, double &eo) {
+        const double vc = q[t][0];
         vo = vc;
-        const double lap = (u[xr] - 2 * vc + u[xl]) * a.ih2 + (u[yr] - 2 * vc + u[yl]) * a.ih2 +
-                           (u[zr] - 2 * vc + u[zl]) * a.ih2;
+        const double lap = (q[t][2] - 2 * vc + q[t][1]) * a.ih2 + (q[t][4] - 2 * vc + q[t][3]) * a.ih2 +
+                           (q[t][6] - 2 * vc + q[t][5]) * a.ih2;
         eo = a.D[f] * lap;
     };
     double vi[NI], ei[NI], di[NI], vb[NB], eb[NB], db[NB];
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         vi[j] = ei[j] = di[j] = 0.0;
-        if (j < a.ni) diffuse(j, vi[j], ei[j]);
+        if (j < a.ni) diffuse(j, j, vi[j], ei[j]);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         vb[b] = eb[b] = db[b] = 0.0;
-        if (b < a.nbb) diffuse(a.ni + b, vb[b], eb[b]);
+        if (b < a.nbb) diffuse(NI + b, a.ni + b, vb[b], eb[b]);
     }
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
