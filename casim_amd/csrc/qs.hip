@@ -212,8 +212,18 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     for (int i = 0; i < n; ++i) total += rate[i];
     double *st = q.stats + (size_t)s * QS_NSTAT;
     if (!(total > 0.0)) {
-        // nothing can happen any more: N stays until t_max
+        // nothing can happen any more: N stays until t_max, and the open
+        // channels keep drawing on the jSR until then
         st[QS_WQ] += st[QS_NCUR] * (q.t_max - fmin(st[QS_TCUR], q.t_max));
+        if (q.deplete && q.t[s] < q.t_max) {
+            const double c0 = q.jsr[s];
+            double geff = 0.0;
+            for (int i = 0; i < n; ++i)
+                if (q.state[off + i] == 1) geff += q.gch * (1.0 - (double)cav[i] / c0);
+            const double cw = jsr_advance(q, c0, geff, q.t_max - q.t[s]);
+            q.jsr[s] = cw;
+            if (cw < q.jsr_min[s]) q.jsr_min[s] = cw;
+        }
         q.active[s] = 0;
         q.waiting[s] = 0;
         return;

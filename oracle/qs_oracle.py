@@ -86,12 +86,19 @@ def event(states, ca, model, params, mg, stream, t, pool=None, t_max=math.inf):
     """One Gillespie event (QuasyStationaryReleaseSite.cpp:101-128) with mouth
     [Ca2+] `ca` for the current states; returns (t, chosen channel).  The
     channel flips (TwoStatePoModel / TwoStateModel::MakeTransition).  With a
-    JsrPool the jSR advances over the interval before the flip."""
+    JsrPool the jSR advances over the interval before the flip.  With every
+    rate zero nothing can happen: the pool advances to t_max and (inf, -1)
+    comes back."""
     n = len(states)
     rates = [oracle.channel_rate(model, params, int(states[i]), float(ca[i]), mg) for i in range(n)]
     total = 0.0
     for r in rates:
         total += r
+    if not total > 0.0:
+        # nothing can happen any more; open channels keep drawing on the jSR to t_max
+        if pool is not None and t < t_max:
+            pool.advance(states, ca, t_max - t, t, t_max)
+        return math.inf, -1
     t_start = t
     dtw = stream.rexp(1.0 / total)
     t = t + dtw
@@ -122,6 +129,8 @@ def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn, poo
     while n_open >= 1 and t < t_max:
         jsr = pool.c if pool is not None else None
         t, idx = event(states, mouth_fn(states, jsr), model, params, mg, stream, t, pool, t_max)
+        if idx < 0:  # no event can happen any more
+            break
         n_open = int(states.sum())
         rec.append((t, states.copy()))
     return rec, stream.state

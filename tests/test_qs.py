@@ -123,6 +123,19 @@ def test_jsr_pool_refill_and_depletion_closed_forms():
     assert math.isclose(p.c, 1000.0 - 1e-3 / beta * 1000.0 * 2e4 * 0.75 * 1e-3, rel_tol=1e-13)
 
 
+def test_no_possible_event_advances_the_pool_to_t_max():
+    """Every rate zero (an open channel that cannot close): no event happens,
+    and the open channel keeps drawing on the jSR until t_max -- the pool ends
+    where one uninterrupted interval of t_max leaves it."""
+    seed = O.stream_states([2024] * 6, 1)[0]
+    pool = Q.JsrPool(1000.0, 2e4, 1e-4, 10000.0, 1.0, 800.0, 100.0, 1000.0)
+    rec, _ = Q.run_site(3, 1, seed, SHEEP, sheep(0.0, 0.0, 0.0), 0.0, 2.0, lambda st, j: np.full(3, 300.0), pool)
+    assert len(rec) == 1  # only the InitOpening record
+    ref = Q.JsrPool(1000.0, 2e4, 1e-4, 10000.0, 1.0, 800.0, 100.0, 1000.0)
+    want = ref._run(1000.0, 2e4 * (1.0 - float(np.float32(300.0)) / 1000.0), 2.0)
+    assert pool.c == want and pool.low == want and want < 990.0
+
+
 def test_waiting_times_are_exponential():
     # constant total rate 5 channels x 0.2 / ms: mean waiting time 1 ms
     seed = O.stream_states([777] * 6, 1)[0]
@@ -270,6 +283,35 @@ def test_depleting_jsr_whole_run_matches_oracle(gpu):
         np.testing.assert_allclose(stats[s], Q.analyze(rec), rtol=1e-12, atol=1e-15)
         assert now[s] == pytest.approx(pool.c, rel=1e-13) and low[s] == pytest.approx(pool.low, rel=1e-13)
     assert low.min() < 990.0  # the pool depletes
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_depleting_jsr_without_events_matches_oracle(gpu):
+    """Rates all zero: no site has an event, and each site's open channel
+    depletes the jSR to t_max as in the oracle."""
+    from casim_amd import surrogate
+    from casim_amd.config import PoolSpec, element_volume
+    spec = _spec(6)
+    for k in ("kopen0", "kclose0"):
+        spec.channel_types[0].params[k] = 0.0
+    net = _net(zero_bias=0.6)
+    ps = PoolSpec(jsr_constant=False, jsr_cq_total=10000.0, jsr_volume=1e6, jsr_c0=[1000.0], jsr_cq_sites=[1.0],
+                  jsr_k_cq=[800.0], jsr_tau_refill=[100.0], nsr_c=[1000.0])
+    qs = surrogate.QuasiStationarySites(spec, np.full(spec.n_sites, 1000.0), 2.0, net, spread_threshold=0.02,
+                                        pools=ps)
+    stats = qs.RunSimulation(fallback=False)
+    now, low = qs.GetJsr()
+    ca = qs.GetMouths()[0]
+    g = spec.channel_types[0].conductance[0]
+    for s, site in enumerate(spec.sites):
+        n = len(site)
+        c = float(ca[_site_rows(qs, s)][0])
+        pool = Q.JsrPool(1000.0, g, element_volume(spec) / 1e6, 10000.0, 1.0, 800.0, 100.0, 1000.0)
+        Q.run_site(n, spec.init_channel[s], spec.rng_states[s], SHEEP, _par(spec), 0.0, 2.0,
+                   lambda st, j: np.full(n, c), pool)
+        assert now[s] == pytest.approx(pool.c, rel=1e-13) and low[s] == pytest.approx(pool.low, rel=1e-13)
+    assert stats[:, 5].sum() == 0 and low.max() < 999.0
     qs.close()
 
 
