@@ -65,7 +65,7 @@ void sweep2_ring(CasimDyad *d, double dt) {
     const int nchunks = (p.nx - 2 + chunk - 1) / chunk;
     const int units = p.S * nchunks;
     const int grid = (units + 7) / 8 * 8;
-    const size_t lds = 4 * (size_t)p.n_grid * P * sizeof(T);
+    const size_t lds = (DIM == 3 ? 4 : 2) * (size_t)p.n_grid * P * sizeof(T);  // k_sweep2's LDS planes
     hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL, RING>), dim3(grid), dim3(d->sweep_threads), lds,
                        d->stream,
                        (const T *)d->g[d->cur], (T *)d->g[d->cur ^ 1], (const T *)d->f[d->cur],
