@@ -398,3 +398,30 @@ def test_gpu_box_conserves_calcium_full_shape(gpu):
     d1, b1 = totals()
     assert np.max(np.abs((d1 + b1) - (d0 + b0)) / (d0 + b0)) < 1e-11
     assert np.min(np.abs(d1 - d0) / (d0 + b0)) > 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_box_full_shape_matches_oracle(gpu, oracle_mod):
+    """The coupled workload's own shapes (2D 201 x 201 dyad, 51^3 box, 50
+    substeps, 64 RyRs, depleting jSR) on 2 sites, fast gating and perturbed
+    fields so that the rim exchange carries flux from the first step: after
+    100 steps (two box steps, pairs of steps per pass) every dyad and box
+    field matches the oracle to 1e-9 and the channel histories are equal."""
+    from casim_amd.dyad import DyadHIP  # noqa: F401
+    say("box full shape vs oracle")
+    sp = workloads.coupled_dyads(2)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    sp.channel_types[0].params["kclose0"] *= 1e3
+    box = workloads.coupled_box(sp)
+    ora, dev = gpu_pair(oracle_mod, sp, box, workloads.coupled_pools(), perturbed=True, seed=9)
+    assert dev.StepsPerPass() == 2
+    box0 = ora.box_field(0).copy()
+    st0 = ora.dyad.channel_states().copy()
+    for _ in range(2):
+        dev.Step(sp.meta["dt_ms"], 50)
+        ora.step(sp.meta["dt_ms"], 50)
+        assert np.array_equal(dev.GetChannelsStates(), ora.dyad.channel_states())
+    compare_all(ora, dev, sp, box, FP64_TOL)
+    assert rel_err(dev.GetPools()[0], ora.jsr()) < FP64_TOL
+    assert not np.array_equal(ora.box_field(0), box0)  # the box took up the rim's flux
+    assert not np.array_equal(ora.dyad.channel_states(), st0)  # channels toggled
