@@ -151,7 +151,8 @@ class DyadHIP:
         """{species name: [site][x][y][z]} (DyadRDCPU.cpp:516-539)."""
         out = {}
         for k, nm in enumerate(self.spec.field_names()):
-            if names is None or nm in names or nm.split("-")[-1] in names:
+            # bound fields are "Ion-Buffer"; buffer names may hold '-' themselves ("Fluo-4")
+            if names is None or nm in names or ("-" in nm and nm.split("-", 1)[1] in names):
                 out[nm] = self.GetField(k)
         return out
 

@@ -211,3 +211,48 @@ def test_device_run_writes_the_oracles_file(gpu, oracle_mod):
                 assert np.array_equal(a, b), name
             else:
                 assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)) < 1e-9, (g, name)
+
+
+def test_reference_parameter_file_drives_the_layout():
+    """The reference's own test input (its "Iterations for ..." strides and
+    "Outputs"): every group is laid out from it, and the 2D dyad datasets
+    named in Outputs are the ones created."""
+    import json
+    from casim_amd import config
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "test_input_Parameters.json")
+    sp, j = config.load_json(golden, n_sites=2)
+    it = output.iterations_from_json(j)
+    assert it == {g: 500 for g in output.GROUPS}
+    pools = config.pools_from_json(j, sp)
+    store = output.MemoryStore(None)
+    out = output.ReferenceOutput(store, sp, j["Outputs"], 1000, it, 4, pools=pools)
+    assert out.idx_max == {g: 3 for g in output.GROUPS}
+    assert out.dims["Dyad"] == [sp.nx, sp.ny] and out.dims["SR"] == [sp.max_channels()]
+    src = StampSource(sp, n_cyto_bound=len([1 for ion in sp.ions for b in list(sp.buffers) + list(pools.cyto_buffers)
+                                            if ion.name in b.kinetics]))
+    out.write(0, 0, src)
+    names = sorted(store["Dyad"].keys())
+    # a buffer's name may itself hold '-' ("Fluo-4"): the ion is the part before the first one
+    assert names and all(n in j["Outputs"] or n.split("-", 1)[1] in j["Outputs"] for n in names)
+    assert "Calcium-Fluo-4" in names
+    assert store["Dyad"][names[0]].shape == (1, 2, sp.nx, sp.ny)
+    assert store["Dyad"][names[0]].maxshape == (3, 4, sp.nx, sp.ny)
+
+
+def test_dyad_getconcentrations_selects_hyphenated_buffers():
+    """DyadHIP.GetConcentrations picks "Ion-Buffer" fields by the buffer's
+    full name, also when that name holds '-' (the reference's "Fluo-4")."""
+    from casim_amd.dyad import DyadHIP
+    from casim_amd import config
+
+    class Fake:
+        spec = None
+
+        def GetField(self, k):
+            return k
+
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "test_input_Parameters.json")
+    f = Fake()
+    f.spec, _ = config.load_json(golden, n_sites=1)
+    got = DyadHIP.GetConcentrations(f, ["Fluo-4"])
+    assert sorted(got) == ["Calcium-Fluo-4"]
