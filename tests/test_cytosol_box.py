@@ -425,3 +425,33 @@ def test_gpu_box_full_shape_matches_oracle(gpu, oracle_mod):
     assert rel_err(dev.GetPools()[0], ora.jsr()) < FP64_TOL
     assert not np.array_equal(ora.box_field(0), box0)  # the box took up the rim's flux
     assert not np.array_equal(ora.dyad.channel_states(), st0)  # channels toggled
+
+
+@pytest.mark.gpu
+def test_gpu_box_profiled_steps_equal_graph_steps(gpu):
+    """casim_dyad_step_profiled (eager launches with HIP events, what the
+    benchmark times) gives the graph path's results bit for bit with a box,
+    in pairs and in single steps."""
+    from casim_amd.dyad import DyadHIP
+    for substeps in (4, 5):
+        sp, box = small_case(dim=2, n_sites=3, clamped=False, far=True, substeps=substeps)
+        sp.channel_types[0].params["kopen0"] *= 1e5
+        devs = []
+        for profiled in (False, True):
+            d = DyadHIP(sp)
+            d.AttachPools(depleting_pool())
+            d.AttachCytosolBox(box)
+            d.InitOpeningAll()
+            d.UpdatePools()
+            for k in (2, 1, 3):
+                if profiled:
+                    d.StepProfiled(2e-6, k * substeps)
+                else:
+                    d.Step(2e-6, k * substeps)
+            devs.append(d)
+        a, b = devs
+        for f in range(sp.n_ions + sp.n_bound):
+            assert np.array_equal(a.GetField(f), b.GetField(f)), (substeps, f)
+        for f in range(a.CytosolBoxFields()):
+            assert np.array_equal(a.GetCytosolBox(f), b.GetCytosolBox(f)), (substeps, f)
+        assert np.array_equal(a.GetChannelsStates(), b.GetChannelsStates())
