@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "simulated dyad-ms/s on 1 GPU (plain vs surrogate) at ≤2% Ca err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-COPY_GBS = 5140.0  # a plain 1 GiB device copy on this MI355X (casim_amd/copy_probe.py, rocprofv3 trace)
+COPY_GBS = 5140.0  # a plain 1 GiB device-to-device copy on MI355X, measured with rocprofv3 (kernel trace)
 
 
 def log(*a):
@@ -83,7 +83,8 @@ def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, re
         "bytes_per_site_step": bps,
         "sweep_kernel_ms": ms_sweep / steps, "channel_kernel_ms": ms_chan / steps,
         "steps_per_launch": per_pass, "sweep_launch_ms": ms_sweep / launches,
-        "sweep_gbs": n_sites * bps / (ms_sweep / steps * 1e-3) / 1e9,
+        # one read and one write of the state per launch, whatever the steps it advances
+        "sweep_gbs": n_sites * bps / (ms_sweep / launches * 1e-3) / 1e9,
         "finite": ok, "open_fraction": open_frac,
         "workload": sp.meta["workload"],
         "tuning": {"planes": tuned["planes"], "threads": tuned["threads"]},
@@ -92,27 +93,45 @@ def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, re
     return res
 
 
-def cpu_baseline(workloads, n_sites=32, steps=5000):
+def cpu_baseline(workloads, n_sites=32, steps=5000, single_sites=4, single_steps=250):
     """The oracle (C restatement of DyadRDCPU, OpenMP over sites as
-    ParallelHybridSolver does) on the first n_sites sites of the same batch."""
+    ParallelHybridSolver::RunSimulationsCPU does, ParallelHybridSolver.cpp:
+    408-437, with the serial Update after the parallel loop) on the first
+    n_sites sites of the same batch, on every host thread; then the same
+    sites on one thread for the single-core cost per site-step."""
     from oracle import oracle as O
     O.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    sp = workloads.batched_dyads(n_sites=4096)
-    sp.sites = sp.sites[:n_sites]
-    sp.rng_states = sp.rng_states[:n_sites]
-    sp.init_channel = sp.init_channel[:n_sites]
-    o = O.Oracle(sp)
-    o.init_opening()
-    ri, rb = sp.rim_arrays()
-    o.update(ri, rb, sp.jsr_array([1000.0]))
-    dt = sp.meta["dt_ms"]
+    dt = workloads.DT_MS
+
+    def sample(n):
+        sp = workloads.batched_dyads(n_sites=4096)
+        sp.sites = sp.sites[:n]
+        sp.rng_states = sp.rng_states[:n]
+        sp.init_channel = sp.init_channel[:n]
+        o = O.Oracle(sp)
+        o.init_opening()
+        ri, rb = sp.rim_arrays()
+        o.update(ri, rb, sp.jsr_array([1000.0]))
+        return o
+
+    o = sample(n_sites)
     o.step(dt, 2, threads)
     t0 = time.perf_counter()
-    o.step(dt, steps, threads)
+    serial = o.step_timed(dt, steps, threads)
     t = time.perf_counter() - t0
+    o1 = sample(single_sites)
+    o1.step(dt, 2, 1)
+    t1 = time.perf_counter()
+    o1.step(dt, single_steps, 1)
+    t1 = time.perf_counter() - t1
     return {"value": n_sites * steps * dt / t, "unit": "dyad-ms/s", "cores": threads, "kind": "port",
-            "sample": "first %d sites of the same 4096-site batch, %d steps (%.1f s wall)" % (n_sites, steps, t)}
+            "sample": "first %d sites of the same 4096-site batch (81x81x5 FP64), %d steps of RunRD + RunMC "
+                      "per site in parallel then the serial Update, %.1f s wall" % (n_sites, steps, t),
+            "us_per_site_step": 1e6 * t / (n_sites * steps),
+            "single_core_us_per_site_step": 1e6 * t1 / (single_sites * single_steps),
+            "single_core_sample": "first %d sites, %d steps on one thread (%.1f s)" % (single_sites, single_steps, t1),
+            "serial_update_fraction": serial / t}
 
 
 def surrogate_leg(workloads, n_sites, t_max, plain_value):
@@ -188,8 +207,11 @@ def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100, tuning=No
            "wall_s": t, "ms_per_step": 1e3 * t / steps,
            "value": n_sites * steps * dt / t, "unit": "dyad-ms/s",
            "sweep_kernel_ms": ms_sweep / chunk, "channel_kernel_ms": ms_chan / chunk,
-           "sweep_gbs": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9,
-           "sweep_frac_of_hbm_peak": n_sites * dev.BytesPerSiteStep() / (ms_sweep / chunk * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           # one read and one write of the state per sweep launch (a launch advances steps_per_pass steps)
+           "sweep_launch_ms": ms_sweep / (chunk / dev.StepsPerPass()),
+           "sweep_gbs": n_sites * dev.BytesPerSiteStep() / (ms_sweep / (chunk / dev.StepsPerPass()) * 1e-3) / 1e9,
+           "sweep_frac_of_hbm_peak": n_sites * dev.BytesPerSiteStep() / (ms_sweep / (chunk / dev.StepsPerPass()) * 1e-3)
+           / 1e9 / HBM_PEAK_GBS,
            "jsr_min_uM": float(jsr.min()), "box_ca_max_uM": float(ca.max()),
            "finite": bool(np.isfinite(ca).all() and np.isfinite(dev.GetField(0)).all())}
     dev.close()
@@ -267,9 +289,12 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(workloads)
         log("cpu:", json.dumps(cpu))
-    # dominant kernel: k_sweep2 (two steps per launch) or k_sweep (one)
+    # dominant kernel: k_sweep2 (two steps per launch) or k_sweep (one).  Its
+    # algorithmic bytes per launch are one read and one write of every
+    # site's state (sites x bytes per site-step): a two-step launch moves the
+    # state once, so the steps it advances do not multiply the bytes.
     kernel = "k_sweep2" if r64["steps_per_launch"] == 2 else "k_sweep"
-    launch_bytes = r64["steps_per_launch"] * a.sites * r64["bytes_per_site_step"]
+    launch_bytes = a.sites * r64["bytes_per_site_step"]
     achieved = launch_bytes / (r64["sweep_launch_ms"] * 1e-3) / 1e9
     traffic = traffic_from_profiles(kernel)
     out = {
@@ -291,12 +316,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "steps_per_launch": r64["steps_per_launch"],
-                     "launch_ms": r64["sweep_launch_ms"],
+                     "launch_ms": r64["sweep_launch_ms"], "algorithmic_bytes_per_launch": launch_bytes,
                      # what HBM actually moves per launch (traffic / launch time), next to the copy rate
                      "physical_gbs": None if traffic is None else traffic / (r64["sweep_launch_ms"] * 1e-3) / 1e9,
-                     "copy_gbs": COPY_GBS},
+                     "copy_gbs": COPY_GBS,
+                     # per-step algorithmic rate: the bytes a one-step-per-pass sweep would move
+                     # for the same steps, over the same time (not bounded by the HBM peak)
+                     "effective_gbs": r64["steps_per_launch"] * achieved},
         "cpu_baseline": cpu,
-        "hbm_gbs_whole_step": a.sites * r64["bytes_per_site_step"] / (r64["ms_per_step"] * 1e-3) / 1e9,
+        # one-step-per-pass bytes of a whole step (channels + sweep) over its time
+        "effective_gbs_whole_step": a.sites * r64["bytes_per_site_step"] / (r64["ms_per_step"] * 1e-3) / 1e9,
         "fp32": None if r32 is None else {
             "value": r32["dyad_ms_per_s"], "ms_per_step": r32["ms_per_step"], "sweep_gbs": r32["sweep_gbs"],
             "sweep_kernel_ms": r32["sweep_kernel_ms"]},

@@ -768,19 +768,26 @@ void oracle_run_mc(OracleDyad *o, double dt, int s) {
     }
 }
 
-void oracle_step(OracleDyad *o, double dt, int k, int n_threads) {
+/* The loop of RunSimulationsCPU with a constant rim (ParallelHybridSolver.cpp:
+ * 408-437): RunRD and RunMC of every site in an OpenMP loop, then the serial
+ * Update.  serial_s (may be NULL) accumulates the wall time of the serial part. */
+void oracle_step_timed(OracleDyad *o, double dt, int k, int n_threads, double *serial_s) {
     for (int it = 0; it < k; ++it) {
 #pragma omp parallel for num_threads(n_threads) schedule(static)
         for (int s = 0; s < o->S; ++s) {
             oracle_run_rd(o, dt, s);
             oracle_run_mc(o, dt, s);
         }
+        double t0 = serial_s ? omp_get_wtime() : 0.0;
         set_mouths(o);
         set_boundaries(o);
         get_gradients(o);
         set_currents(o);
+        if (serial_s) *serial_s += omp_get_wtime() - t0;
     }
 }
+
+void oracle_step(OracleDyad *o, double dt, int k, int n_threads) { oracle_step_timed(o, dt, k, n_threads, NULL); }
 
 int oracle_max_channels(const OracleDyad *o) { return o->nch_max; }
 int oracle_site_channels(const OracleDyad *o, int s) { return o->ch_off[s + 1] - o->ch_off[s]; }
@@ -1167,6 +1174,20 @@ void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *boun
 }
 
 int oracle_hybrid_cyto_bound(const OracleHybrid *h) { return h->nBc; }
+
+/* Overwrite the cytosol pool's ions [site + ion*S] and bound fields
+ * [site + b*S] (either may be NULL); free buffers and the dyad-side boundary
+ * values follow (CytosolPool::_GetFreeBuffers).  For tests. */
+void oracle_hybrid_set_cytosol(OracleHybrid *h, const double *ions, const double *bound) {
+    if (ions) memcpy(h->cions, ions, sizeof(double) * h->ni * h->S);
+    if (bound) memcpy(h->cbuf, bound, sizeof(double) * h->nBc * h->S);
+    for (int s = 0; s < h->S; ++s) {
+        int kd = 0;
+        for (int b = 0; b < h->nBc; ++b)
+            if (h->cb_dyad[b]) h->cbb[s + (kd++) * h->S] = h->cbuf[s + b * h->S];
+        cyto_free(h, s);
+    }
+}
 
 /* ------------------------------------------------------------------------- */
 /* Extension: a 3D cytosol box around each site, coupled at the dyad's rim.

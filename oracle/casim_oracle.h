@@ -59,6 +59,7 @@ void oracle_run_mc(OracleDyad *o, double dt, int site);
 /* k steps of {RunRD(site), RunMC(site) for every site (OpenMP, n_threads)}
  * followed by Update with the rim/jSR of the last oracle_update. */
 void oracle_step(OracleDyad *o, double dt, int k, int n_threads);
+void oracle_step_timed(OracleDyad *o, double dt, int k, int n_threads, double *serial_s);
 
 int oracle_max_channels(const OracleDyad *o);
 int oracle_site_channels(const OracleDyad *o, int site);
@@ -91,6 +92,7 @@ void oracle_hybrid_step(OracleHybrid *h, double dt, int k, int n_threads);
 void oracle_hybrid_get_jsr(const OracleHybrid *h, double *out);
 void oracle_hybrid_get_cytosol(const OracleHybrid *h, double *ions, double *bound);
 int oracle_hybrid_cyto_bound(const OracleHybrid *h);
+void oracle_hybrid_set_cytosol(OracleHybrid *h, const double *ions, const double *bound);
 
 /* Extension: the 3D cytosol box of CasimCytosolBoxConfig (casim_dyad.h),
  * the same coupling order as casim_dyad_step.  Returns 0, or -1 for a

@@ -48,6 +48,7 @@ def load():
             "oracle_run_rd": (None, [P, C.c_double, C.c_int]),
             "oracle_run_mc": (None, [P, C.c_double, C.c_int]),
             "oracle_step": (None, [P, C.c_double, C.c_int, C.c_int]),
+            "oracle_step_timed": (None, [P, C.c_double, C.c_int, C.c_int, _dp]),
             "oracle_max_channels": (C.c_int, [P]),
             "oracle_site_channels": (C.c_int, [P, C.c_int]),
             "oracle_channel_node": (C.c_int, [P, C.c_int, C.c_int]),
@@ -71,6 +72,7 @@ def load():
             "oracle_hybrid_get_jsr": (None, [P, _dp]),
             "oracle_hybrid_get_cytosol": (None, [P, _dp, _dp]),
             "oracle_hybrid_cyto_bound": (C.c_int, [P]),
+            "oracle_hybrid_set_cytosol": (None, [P, _dp, _dp]),
             "oracle_hybrid_attach_box": (C.c_int, [P, C.c_void_p]),
             "oracle_hybrid_box_fields": (C.c_int, [P]),
             "oracle_hybrid_get_box": (None, [P, C.c_int, _dp]),
@@ -158,6 +160,12 @@ class Oracle:
 
     def step(self, dt, k, threads=None):
         self.lib.oracle_step(self.h, dt, k, threads or os.cpu_count() or 1)
+
+    def step_timed(self, dt, k, threads=None):
+        """step(); returns the wall seconds spent in the serial Update part."""
+        t = np.zeros(1)
+        self.lib.oracle_step_timed(self.h, dt, k, threads or os.cpu_count() or 1, d(t))
+        return float(t[0])
 
     def field(self, f):
         out = np.empty(self.NS * self.S)
@@ -296,3 +304,9 @@ class OracleHybridRun:
         bound = np.empty(max(nb, 1) * self.spec.n_sites)
         self.lib.oracle_hybrid_get_cytosol(self.h, d(ions), d(bound))
         return ions.reshape(self.spec.n_ions, -1), bound[:nb * self.spec.n_sites].reshape(nb, -1)
+
+    def set_cytosol(self, ions=None, bound=None):
+        """Overwrite the cytosol pool's ions [ion, site] and bound fields [b, site]."""
+        ia = None if ions is None else np.ascontiguousarray(ions, dtype=np.float64).reshape(-1)
+        ba = None if bound is None else np.ascontiguousarray(bound, dtype=np.float64).reshape(-1)
+        self.lib.oracle_hybrid_set_cytosol(self.h, None if ia is None else d(ia), None if ba is None else d(ba))
