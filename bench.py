@@ -134,27 +134,61 @@ def cpu_baseline(workloads, n_sites=32, steps=5000, single_sites=4, single_steps
             "serial_update_fraction": serial / t}
 
 
-def surrogate_leg(workloads, n_sites, t_max, plain_value):
+def surrogate_leg(workloads, DyadHIP, n_sites, t_max, plain_value, heldout_sites=256, ks_sites=1024, ks_tmax=1.0):
     """The surrogate-driven quasi-stationary path (the event-driven release
     site with the learned mouth-[Ca2+] ensemble) on the same sites and
     kinetics as the plain leg: every site from rest, its init channel open,
-    to t_max, jSR constant.  Throughput in the plain leg's unit."""
+    to t_max, jSR constant; throughput in the plain leg's unit.  Measured in
+    the same run: the ensemble's mouth-[Ca2+] error on held-out geometries
+    that no training batch used, solved by the full FP64 solver; and, at a
+    t_max the plain path also runs (ks_tmax, depleting jSR, ks_sites sites),
+    both paths' throughput and the KS comparison of their release statistics."""
     w = os.path.join(ROOT, "casim_amd", "surrogate_weights.npz")
     if not os.path.exists(w):
         return None
-    from casim_amd import mlp, spark_compare, surrogate
+    from casim_amd import autotune, mlp, spark_compare, surrogate, train_surrogate
     net, lin = surrogate.load_model(w)
     spec = workloads.batched_dyads(n_sites=n_sites)
     spark_compare.run_qs(spec, net, 0.05, lin=lin)  # warm-up: workspaces and kernels loaded
     win, full, wall, fb = spark_compare.run_qs(spec, net, t_max, lin=lin)
     meta = mlp.MLP.meta(w)
-    held = meta.get("heldout", {})
     value = n_sites * t_max / wall
-    return {"value": value, "unit": "dyad-ms/s", "speedup_vs_plain": value / plain_value, "t_max_ms": t_max,
-            "wall_s": wall, "events_per_site": float(full[:, 5].mean()), "fallback_sites": int(fb),
-            "rounds": spark_compare.run_qs.last["rounds"], "fallback_s": spark_compare.run_qs.last["fallback_s"],
-            "mouth_ca_rel_err_heldout": {k: held.get(k) for k in ("rel_mean", "rel_p50", "rel_p95", "rel_mean_above_5uM")},
-            "target": meta.get("target")}
+    out = {"value": value, "unit": "dyad-ms/s", "speedup_vs_plain": value / plain_value, "t_max_ms": t_max,
+           "wall_s": wall, "events_per_site": float(full[:, 5].mean()), "fallback_sites": int(fb),
+           "rounds": spark_compare.run_qs.last["rounds"], "fallback_s": spark_compare.run_qs.last["fallback_s"],
+           "target": meta.get("target")}
+    # held-out error, in this run: fresh geometries, open sets and jSR values
+    t = autotune.lookup(workloads.batched_dyads(n_sites=heldout_sites, clamped=True))
+    tuning = (t["planes"], t["threads"], t.get("prefetch", 1)) if t else None
+    t0 = time.perf_counter()
+    xh, yh, _ = surrogate.generate(DyadHIP, n_sites=heldout_sites, seed=50_000, tuning=tuning, lin=lin)
+    held = train_surrogate.errors(net, xh, yh, lin=lin)
+    out["mouth_ca_rel_err_heldout"] = {"sites": heldout_sites, "seconds": round(time.perf_counter() - t0, 1),
+                                       **{k: held.get(k) for k in ("rows", "rel_mean", "rel_p50", "rel_p95",
+                                                                   "rel_mean_above_5uM", "flagged_frac")}}
+    # both paths at one t_max, depleting jSR: throughput and KS of the release statistics
+    kspec = spark_compare.workload(ks_sites, kinetics="bench")
+    n_ch = np.array([len(st) for st in kspec.sites])
+    pools = spark_compare.depleting_pool()
+    kwin, _, kwq, kfb = spark_compare.run_qs(kspec, net, ks_tmax, pools=pools, lin=lin)
+    jq = spark_compare.run_qs.jsr_low
+    kplain, kwp = spark_compare.run_plain(kspec, ks_tmax, pools=pools)
+    jp = spark_compare.run_plain.jsr_low
+    ks = spark_compare.compare(kplain, kwin, n_ch, ks_tmax)
+    dt = kspec.meta["dt_ms"]
+    po = [spark_compare.on_grid(a, dt) / (n_ch * ks_tmax) for a in (kplain[:, 3], kwin[:, 2])]
+    jr = spark_compare.compare_jsr_lowest(kplain, kwin, jp, jq)
+    out["same_tmax"] = {
+        "sites": ks_sites, "t_max_ms": ks_tmax, "jsr": "depleting (SRPool)",
+        "plain_dyad_ms_per_s": ks_sites * ks_tmax / kwp, "surrogate_dyad_ms_per_s": ks_sites * ks_tmax / kwq,
+        "speedup": kwp / kwq, "fallback_sites": int(kfb),
+        "ks_p": {"amplitude": ks["amplitude"]["p"], "last_event": ks["last_event_ms"]["p"],
+                 "open_integral": ks["open_integral"]["p"], "n_open_end": ks["n_open_end"]["p"],
+                 "open_probability": spark_compare.analysis.ks_2samp(po[0], po[1])[1],
+                 "jsr_lowest_other_sites": jr["others"]["p"]},
+        "jsr_lowest_held_open": {"weight_p": jr["held_open_weight_p"], "drop_rel_diff": jr.get("held_open_drop_rel_diff")},
+        "classes_plain": ks["classes_plain"], "classes_qs": ks["classes_qs"]}
+    return out
 
 
 def coupled_leg(workloads, DyadHIP, n_sites, steps, warmup, chunk=100, tuning=None):
@@ -277,7 +311,7 @@ def main():
     sur = None
     if not a.no_surrogate:
         log("surrogate leg")
-        sur = surrogate_leg(workloads, a.sites, a.qs_tmax, r64["dyad_ms_per_s"])
+        sur = surrogate_leg(workloads, DyadHIP, a.sites, a.qs_tmax, r64["dyad_ms_per_s"])
         log("surrogate:", json.dumps(sur))
     cpl = None
     if not a.no_coupled:

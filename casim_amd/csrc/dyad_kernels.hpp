@@ -53,6 +53,8 @@ struct Channels {
     uint32_t *rng;      // [S*6]
     const uint32_t *jump;  // [JUMP_N*18]
     double *mouths;     // [K*S*M]  K = ni + nb + nu
+    double *snap;       // [(n_grid + n_face) * nch] step-k+1 values of the mouth nodes, node-sorted,
+                        // stored by k_sweep2 (face values 0 off the two z-faces)
     const double *jsr;  // [M*S*ni]
     double *sJ;         // [ni*nch] currents in node-sorted order
     double *totals;     // [ni*S]
@@ -417,12 +419,14 @@ template <int LY> __device__ __forceinline__ double ly_total(const Params &p, in
 
 // Updated values of one interior node in DyadRDCPU.cpp:404-492 order: the
 // Laplacian of every diffusing field (x-neighbours from registers, y and z
-// from the LDS plane L), the reactions with the free buffers seen by each
+// from the LDS plane), the reactions with the free buffers seen by each
 // bound field, the channel current, the sarcolemmal factor at z = nz-1 (3D)
 // or everywhere (2D), the Euler step.  C0/Cm/Cp: the node and its x-1 / x+1
-// neighbours, every grid field; Fv: face values of the node.  Writes the new
-// grid values to G (every grid field) and, at face nodes, the new face values
-// to Fo.
+// neighbours, every grid field; Fv: face values of the node.  The LDS plane
+// is node-major ([node][grid field]): Lc points at the node's first field,
+// its y-neighbours sit sy elements away and its z-neighbours zl, zr elements
+// away (the mirror image on the z-faces).  Writes the new grid values to G
+// (every grid field) and, at face nodes, the new face values to Fo.
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int LI_ = NI ? NI : MAXI>
 __device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], const T (&C0)[NG],
                                             const T (&Cp)[NG], const T *L, int c, int P, int zl, int zr, int z,
@@ -721,9 +725,9 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 // state per two steps.  The currents of step k+2 are those the channel kernel
 // would compute from the step-k+1 mouth values and the states it gated at
 // step k+1 (already in state[]): J = state * g * (jSR - c_mouth), the last
-// channel on a node winning; the mouth values of step k+1 (ions, bound
-// fields, free buffers) are written to mouths[] for the next channel call,
-// which gates step k+2 from them.  Rim nodes stay at their step-k values
+// channel on a node winning; the step-k+1 values of every mouth node are
+// written to snap[] for the next channel call, which derives the mouth
+// snapshot (ions, bound fields, free buffers) from them and gates step k+2.  Rim nodes stay at their step-k values
 // (constant Dirichlet rim: no cytosol pool); with a jSR pool the currents use
 // the pool's value, which the channel call before the pass has advanced.  Chunks of one site recompute one halo plane of
 // step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
@@ -769,6 +773,13 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
 
     PlaneSlots<DIM, SL> ps;
     ps.init(p, P);
+    // per-lane slot flags (bit 0: slot on the plane, bit 1: interior node,
+    // bit 2: z-face node), re-read each plane through an empty asm so that
+    // the compiler tests them where they are used instead of keeping one
+    // 64-bit lane mask per flag live across the plane loop
+    int fl[SL];
+#pragma unroll
+    for (int k = 0; k < SL; ++k) fl[k] = (ps.act[k] ? 1 : 0) | (ps.inner[k] ? 2 : 0) | (ps.fz[k] >= 0 ? 4 : 0);
 
     T R[4][SL][NG];   // step k: planes x-1, x, x+1, x+2 (prefetch)
     T F[2][SL][NF];   // step k faces: planes x, x+1 (prefetch)
@@ -779,7 +790,7 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && ps.act[k]) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
+                if (g < ng && (fl[k] & 1)) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
     };
     auto load_face = [&](T (&D)[SL][NF], int x) {
 #pragma unroll
@@ -804,6 +815,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
     const int off = ch_off[s];
 
     for (int x = xa0; x <= x1; ++x) {
+#pragma unroll
+        for (int k = 0; k < SL; ++k) asm volatile("" : "+v"(fl[k]));
         if (x + 2 <= min(x1 + 1, p.nx - 1)) load_plane(R[3], x + 2);
         if (x + 1 <= x1) load_face(F[1], x + 1);
         const bool doB = x - 1 >= x0;
@@ -816,7 +829,7 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && ps.act[k]) {
+                if (g < ng && (fl[k] & 1)) {
                     if (interiorA) La[g * P + ps.q[k]] = R[1][k][g];
                     if (doB) Lb[g * P + ps.q[k]] = A[1][k][g];
                 }
@@ -844,11 +857,11 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
             }
 #pragma unroll
             for (int k = 0; k < SL; ++k) {
-                if (interiorA && ps.inner[k]) {
+                if (interiorA && (fl[k] & 2)) {
                     const int c = ps.q[k];
                     const int z = DIM == 3 ? c % p.nz : 0;
                     node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], La, c, P, ps.zl[k], ps.zr[k],
-                                                        z, ps.fz[k] >= 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2,
+                                                        z, (fl[k] & 4) != 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2,
                                                         A[2][k], FA[1][k]);
                 } else {
 #pragma unroll
@@ -892,73 +905,43 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
 #pragma unroll
                 for (int k = 0; k < SL; ++k)
                     if (ps.q[k] == qq) {
-                        // the channel kernel's snapshot and current (k_channels), step k+1 values
+                        // the channel kernel's current (k_channels) from the step-k+1
+                        // mouth value; the node's step-k+1 values go to the snapshot
+                        // the next channel call reads (CH_FROM_STORED)
                         const int st = cstate[off + i];
                         const int typ = ctype[off + i];
-                        const int z = DIM == 3 ? qq % p.nz : 0;
-                        const bool fc = DIM == 3 && (z == 0 || z == p.nz - 1);
-                        double bnd[LB], fre[MAXU];
-#pragma unroll
-                        for (int b = 0; b < LB; ++b) {
-                            if (!NB && b >= p.nb) break;
-                            double v = 0.0;
-                            const int g = ly_bslot<LY>(p, b);
-                            if (ly_bface<LY>(p, b)) {
-#pragma unroll
-                                for (int f = 0; f < NF; ++f)
-                                    if (f == g && fc) v = (double)FA[0][k][f];
-                            } else {
-#pragma unroll
-                                for (int gg = 0; gg < NG; ++gg)
-                                    if (gg == g) v = (double)A[1][k][gg];
-                            }
-                            bnd[b] = v;
-                        }
-                        // free buffers: total minus bound, subtracted in bound order
-#pragma unroll
-                        for (int u = 0; u < Layout<LY>::nu; ++u) {
-                            double v = u < p.nu ? ly_total<LY>(p, u, z) : 0.0;
-#pragma unroll
-                            for (int b = 0; b < LB; ++b) {
-                                if (!NB && b >= p.nb) break;
-                                if (ly_bbuf<LY>(p, b) == u) v -= bnd[b];
-                            }
-                            fre[u] = v;
-                        }
-                        double *mo = mouths + i;
-                        const size_t MS = (size_t)a.M * p.S;
 #pragma unroll
                         for (int j = 0; j < LI; ++j) {
                             if (!NI && j >= p.ni) break;
                             const double ion = (double)A[1][k][j];
-                            mo[(size_t)a.M * s + j * MS] = ion;
                             // jSR of the step: the device pool's value when pools are on
                             const double jv = jnew ? jnew[s + j * p.S] : jsr[i + (size_t)a.M * (s + (size_t)j * p.S)];
                             const double cond = a.g[typ * MAXI + j] * (jv - ion);
                             J[k][j] = (T)((double)st * cond);
                         }
+                        double *sn = a.snap + curB;
 #pragma unroll
-                        for (int b = 0; b < LB; ++b)
-                            if (b < p.nb) mo[(size_t)a.M * s + (p.ni + b) * MS] = bnd[b];
+                        for (int g = 0; g < NG; ++g)
+                            if (g < ng) sn[(size_t)g * a.nch] = (double)A[1][k][g];
 #pragma unroll
-                        for (int u = 0; u < Layout<LY>::nu; ++u)
-                            if (u < p.nu) mo[(size_t)a.M * s + (p.ni + p.nb + u) * MS] = fre[u];
+                        for (int f = 0; f < NF; ++f)
+                            if (f < nfc) sn[(size_t)(ng + f) * a.nch] = (fl[k] & 4) ? (double)FA[0][k][f] : 0.0;
                     }
                 ++curB;
             }
 #pragma unroll
             for (int k = 0; k < SL; ++k) {
-                if (!ps.inner[k]) continue;
+                if (!(fl[k] & 2)) continue;
                 const int c = ps.q[k];
                 const int z = DIM == 3 ? c % p.nz : 0;
                 T G[NG], Fo[NF];
                 node_update<T, DIM, NI, NB, NG, NF, LY>(p, A[0][k], A[1][k], A[2][k], Lb, c, P, ps.zl[k], ps.zr[k], z,
-                                                    ps.fz[k] >= 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
+                                                    (fl[k] & 4) != 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
                 const size_t o = (size_t)xb * P + c;
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
                     if (g < ng) os[g * fs + o] = G[g];
-                if (ps.fz[k] >= 0)
+                if (fl[k] & 4)
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
                         if (f < nfc)
@@ -1085,32 +1068,28 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
         int st = valid ? a.state[off + i] : 0;
         const int typ = valid ? a.type[off + i] : 0;
         double ion[MAXI], bnd[MAXB], fre[MAXU];
-        if (mode & CH_FROM_STORED) {
-            // the snapshot k_sweep2 wrote for these channels
-            const size_t MS = (size_t)a.M * p.S, base = (size_t)i + (size_t)a.M * s;
-#pragma unroll
-            for (int j = 0; j < MAXI; ++j) ion[j] = (j < p.ni && valid) ? a.mouths[base + j * MS] : 0.0;
-#pragma unroll
-            for (int b = 0; b < MAXB; ++b) bnd[b] = (b < p.nb && valid) ? a.mouths[base + (p.ni + b) * MS] : 0.0;
-#pragma unroll
-            for (int u = 0; u < MAXU; ++u)
-                fre[u] = (u < p.nu && valid) ? a.mouths[base + (p.ni + p.nb + u) * MS] : 0.0;
-        } else if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE)) {
+        if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE | CH_FROM_STORED)) {
             const int node = valid ? a.node[off + i] : 0;
             const int z = p.dim == 3 ? node % p.nz : 0;
             const int xy = p.dim == 3 ? node / p.nz : node;
             const size_t c = (size_t)s * p.NS + node;
+            // CH_FROM_STORED: the node's values that k_sweep2 stored (node-sorted position)
+            const double *sn = a.snap + off + (valid ? a.perm[off + i] : 0);
+            const bool stored = (mode & CH_FROM_STORED) != 0;
 #pragma unroll
-            for (int j = 0; j < MAXI; ++j) ion[j] = (j < p.ni) ? (double)g[j * fs + c] : 0.0;
+            for (int j = 0; j < MAXI; ++j)
+                ion[j] = (j < p.ni) ? (stored ? sn[(size_t)j * a.nch] : (double)g[j * fs + c]) : 0.0;
 #pragma unroll
             for (int b = 0; b < MAXB; ++b) {
                 if (b >= p.nb) {
                     bnd[b] = 0.0;
                 } else if (p.bface[b]) {
                     bool fc = p.dim == 3 && (z == 0 || z == p.nz - 1);
-                    bnd[b] = fc ? (double)load_face(f, p, p.bslot[b], s, z == 0 ? 0 : 1, xy) : 0.0;
+                    bnd[b] = fc ? (stored ? sn[(size_t)(p.n_grid + p.bslot[b]) * a.nch]
+                                          : (double)load_face(f, p, p.bslot[b], s, z == 0 ? 0 : 1, xy))
+                                : 0.0;
                 } else {
-                    bnd[b] = (double)g[p.bslot[b] * fs + c];
+                    bnd[b] = stored ? sn[(size_t)p.bslot[b] * a.nch] : (double)g[p.bslot[b] * fs + c];
                 }
             }
 #pragma unroll
@@ -1118,7 +1097,7 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
 #pragma unroll
             for (int b = 0; b < MAXB; ++b)
                 if (b < p.nb) fre[p.bbuf[b]] -= bnd[b];
-            if ((mode & CH_SNAPSHOT) && valid) {
+            if ((mode & (CH_SNAPSHOT | CH_FROM_STORED)) && valid) {
                 for (int k = 0; k < p.ni; ++k) a.mouths[i + (size_t)a.M * (s + (size_t)k * p.S)] = ion[k];
                 for (int k = 0; k < p.nb; ++k)
                     a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni) * p.S)] = bnd[k];

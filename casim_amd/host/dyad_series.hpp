@@ -46,6 +46,15 @@ class DyadSeriesApi {
     virtual std::map<std::string, std::vector<double>> GetConcentrations(std::vector<std::string> &names) = 0;
     virtual std::map<std::string, std::vector<int>> GetChannelsStates(std::vector<std::string> &names) = 0;
     virtual std::map<std::string, std::vector<double>> GetIonsNearChannels(std::vector<std::string> &names) = 0;
+    // Output dataset shapes (DyadSeries.h:32-40): [time, site, nx, ny(, nz)] for the grid fields,
+    // [time, site, channels] for channel states and mouth values; entry 0 (time) is left 0 for the
+    // writer to fill, as DyadRDCPU's constructor leaves it (DyadRDCPU.cpp:21-25,137-142).
+    std::vector<uint64_t> GetDimensions() { return dyad_dims; }
+    std::vector<uint64_t> GetChannelsDimensions() { return channels_dims; }
+    std::vector<uint64_t> GetIonsNearChannelsDimensions() { return channels_ions_dims; }
+
+  protected:
+    std::vector<uint64_t> dyad_dims, channels_ions_dims, channels_dims;
 };
 
 // A batch of sites on the GPU behind DyadSeriesApi.  The configuration is the
@@ -59,6 +68,11 @@ class DyadRDHIP : public DyadSeriesApi {
           radius_(cfg.radius) {
         check(casim_dyad_create(&cfg, &d_));
         M_ = casim_dyad_max_channels(d_);
+        // DyadRDCPU.cpp:21-25 (3D) / DyadRD2DCPU.cpp:19-22 (2D), and DyadRDCPU.cpp:137-142
+        dyad_dims = {0, (uint64_t)S_, (uint64_t)cfg.nx, (uint64_t)cfg.ny};
+        if (cfg.dim == 3) dyad_dims.push_back((uint64_t)cfg.nz);
+        channels_ions_dims = {0, (uint64_t)S_, (uint64_t)M_};
+        channels_dims = {0, (uint64_t)S_, (uint64_t)M_};
     }
     ~DyadRDHIP() override { casim_dyad_destroy(d_); }
     DyadRDHIP(const DyadRDHIP &) = delete;

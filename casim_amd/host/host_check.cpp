@@ -4,7 +4,9 @@
 //      Update) through the DyadRDHIP adapter equals the fused path
 //      (casim_dyad_step, one captured graph) bit for bit: every field, the
 //      channel states and every site's RNG state, with fast gating;
-//   2. ABI errors arrive as exceptions carrying casim_last_error().
+//   2. ABI errors arrive as exceptions carrying casim_last_error();
+//   3. the output dataset shapes (GetDimensions & co.) are DyadRDCPU's, and
+//      GetConcentrations returns exactly the fields asked for.
 // Workload: 4 sites of a 3D cleft 80 x 80 x 16 nm at h = 4 nm (21 x 21 x 5),
 // Ca2+ with mobile buffers A, B and the membrane buffer F on the z-faces,
 // 3 RyRs per site with SheepCannell gating sped up so that channels toggle.
@@ -154,10 +156,21 @@ int main() {
         msg = e.what();
         error_reported = msg.find("before the first update") != std::string::npos;
     }
-    const bool ok = fields_equal && sa == sf && ra == rf && rng_moved && error_reported;
+    // output dataset shapes the solver copies for its HDF5 writer (DyadRDCPU.cpp:21-25,137-142)
+    const bool dims_ok = proto.GetDimensions() == std::vector<uint64_t>{0, (uint64_t)S, 21, 21, 5} &&
+                         proto.GetChannelsDimensions() == std::vector<uint64_t>{0, (uint64_t)S, (uint64_t)M} &&
+                         proto.GetIonsNearChannelsDimensions() == std::vector<uint64_t>{0, (uint64_t)S, (uint64_t)M};
+    // GetConcentrations returns the fields asked for: an ion by name, bound fields by buffer name
+    std::vector<std::string> some{"Calcium", "B"};
+    auto sel = proto.GetConcentrations(some);
+    const bool names_ok = sel.size() == 2 && sel.count("Calcium") && sel.count("Calcium-B") &&
+                          sel["Calcium"].size() == (size_t)S * 21 * 21 * 5;
+    const bool ok = fields_equal && sa == sf && ra == rf && rng_moved && error_reported && dims_ok && names_ok;
     std::printf("{\"fields_equal\": %s, \"states_equal\": %s, \"rng_equal\": %s, \"channels_changed\": %d, "
-                "\"error_reported\": %s, \"steps\": %d, \"sites\": %d, \"ok\": %s}\n",
+                "\"error_reported\": %s, \"dims_ok\": %s, \"names_ok\": %s, \"steps\": %d, \"sites\": %d, "
+                "\"ok\": %s}\n",
                 fields_equal ? "true" : "false", sa == sf ? "true" : "false", ra == rf ? "true" : "false", toggled,
-                error_reported ? "true" : "false", steps, S, ok ? "true" : "false");
+                error_reported ? "true" : "false", dims_ok ? "true" : "false", names_ok ? "true" : "false", steps, S,
+                ok ? "true" : "false");
     return ok ? 0 : 1;
 }

@@ -33,3 +33,4 @@ def test_host_check_protocol_equals_fused(gpu):
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert r.returncode == 0 and out["ok"], out
     assert out["channels_changed"] > 0
+    assert out["dims_ok"] and out["names_ok"], out
