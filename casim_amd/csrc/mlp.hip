@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <mutex>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "../../include/casim_surrogate.h"
@@ -343,6 +345,75 @@ float mlp_out_std(const CasimMLP *m) { return m->ys; }
 
 }  // namespace casim
 
+// ---------------------------------------------------------------------------
+// Ensemble prediction that allocates nothing: the activations live in a fixed
+// workspace, one per GPU, held from the library's first use of that GPU until
+// the process ends.  Layout for up to CASIM_MLP_NOALLOC_MAX_ROWS rows: two BF16
+// activation buffers [rows x CASIM_MLP_NOALLOC_MAX_WIDTH] (ping-pong), one FP32
+// pre-activation buffer [rows x CASIM_MLP_NOALLOC_MAX_WIDTH], and the members'
+// outputs [CASIM_MLP_NOALLOC_MAX_MEMBERS x rows].
+
+namespace casim {
+namespace {
+constexpr size_t NA_ACT = (size_t)CASIM_MLP_NOALLOC_MAX_ROWS * CASIM_MLP_NOALLOC_MAX_WIDTH;  // elements per activation buffer
+constexpr size_t NA_OFF_XB = NA_ACT * sizeof(uint16_t);
+constexpr size_t NA_OFF_Z = 2 * NA_ACT * sizeof(uint16_t);
+constexpr size_t NA_OFF_OUT = NA_OFF_Z + NA_ACT * sizeof(float);
+constexpr size_t NA_BYTES = NA_OFF_OUT + (size_t)CASIM_MLP_NOALLOC_MAX_MEMBERS * CASIM_MLP_NOALLOC_MAX_ROWS * sizeof(float);
+static_assert(NA_OFF_XB % 256 == 0 && NA_OFF_Z % 256 == 0 && NA_OFF_OUT % 256 == 0, "workspace alignment");
+static_assert(NA_BYTES == CASIM_MLP_NOALLOC_BYTES, "workspace size as declared in casim_surrogate.h");
+
+// The workspace.  In host code this name stands for the symbol only; its
+// device address comes from hipGetSymbolAddress (noalloc_workspace below).
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+void *na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of this GPU's workspace, looked up once per device with that
+// device current.
+void *noalloc_workspace(int dev) {
+    REQUIRE(dev >= 0 && dev < NA_MAX_DEVICES, CASIM_E_DEVICE, "device index out of range for the MLP workspace");
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Err(CASIM_E_DEVICE, std::string("hipGetSymbolAddress of the MLP workspace: ") + hipGetErrorString(e));
+        na_ptr[dev] = ptr;
+    }
+    return na_ptr[dev];
+}
+
+// out[e * cap + r] = z[r] (the output layer's column) for member e.
+__global__ void k_na_collect(const float *__restrict__ z, float *__restrict__ out, int rows) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) out[r] = z[r];
+}
+
+// Mean and relative spread of exp(y * ys + ym) over the members, in the
+// arithmetic of casim_mlp_predict's host reduction.
+__global__ void k_na_reduce(const float *__restrict__ out, int members, int cap, int rows, float ys, float ym,
+                            float *__restrict__ mean, float *__restrict__ spread) {
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double s = 0, s2 = 0;
+    for (int e = 0; e < members; ++e) {
+        const float o = out[(size_t)e * cap + r] * ys + ym;
+        const double v = exp((double)o);
+        s += v;
+        s2 += v * v;
+    }
+    const double mu = s / members;
+    const double var = fmax(0.0, s2 / members - mu * mu);
+    if (mean) mean[r] = (float)mu;
+    if (spread) spread[r] = (float)(mu > 0 ? sqrt(var) / mu : 0.0);
+}
+}  // namespace
+}  // namespace casim
+
 extern "C" {
 
 int casim_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t members, uint64_t seed, CasimMLP **out) {
@@ -353,6 +424,7 @@ int casim_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t members
         m = new CasimMLP();
         HIPCK(hipGetDevice(&m->dev));
         HIPCK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        (void)casim::noalloc_workspace(m->dev);  // this GPU's workspace of casim_mlp_predict_noalloc
         m->n_in = n_in;
         m->width = width;
         m->depth = depth;
@@ -618,6 +690,57 @@ int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, fl
             if (spread) spread[i] = (float)(mu > 0 ? std::sqrt(var) / mu : 0.0);
         }
         if (member_out) std::memcpy(member_out, out.data(), sizeof(float) * out.size());
+    });
+}
+
+int casim_mlp_predict_noalloc(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, void *stream) {
+    return guard([&] {
+        REQUIRE(m && x && rows >= 1, CASIM_E_INVALID, "bad arguments");
+        REQUIRE(rows <= CASIM_MLP_NOALLOC_MAX_ROWS, CASIM_E_RANGE,
+                "rows exceed the workspace of casim_mlp_predict_noalloc (" + std::to_string(CASIM_MLP_NOALLOC_MAX_ROWS) +
+                    "); split the call");
+        REQUIRE(m->width <= CASIM_MLP_NOALLOC_MAX_WIDTH && m->n_in <= CASIM_MLP_NOALLOC_MAX_WIDTH, CASIM_E_RANGE,
+                "layer wider than the workspace of casim_mlp_predict_noalloc (" +
+                    std::to_string(CASIM_MLP_NOALLOC_MAX_WIDTH) + ")");
+        REQUIRE(m->members <= CASIM_MLP_NOALLOC_MAX_MEMBERS, CASIM_E_RANGE,
+                "more members than the workspace of casim_mlp_predict_noalloc holds (" +
+                    std::to_string(CASIM_MLP_NOALLOC_MAX_MEMBERS) + ")");
+        REQUIRE(m->mean && m->istd, CASIM_E_STATE, "normalisation not set");
+        m->use();
+        hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+        unsigned char *ws = (unsigned char *)casim::noalloc_workspace(m->dev);
+        uint16_t *xa = (uint16_t *)ws;
+        uint16_t *xb = (uint16_t *)(ws + casim::NA_OFF_XB);
+        float *z = (float *)(ws + casim::NA_OFF_Z);
+        float *outm = (float *)(ws + casim::NA_OFF_OUT);
+        const int B = (int)rows;
+        const int cap = CASIM_MLP_NOALLOC_MAX_ROWS;
+        const size_t n = (size_t)B * m->n_in;
+        for (int e = 0; e < m->members; ++e) {
+            hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, (const float *)nullptr,
+                               (const int *)nullptr, B, m->n_in, m->mean, m->istd, 0.f, 1.f, xa, (float *)nullptr);
+            HIPCK(hipGetLastError());
+            uint16_t *in = xa, *nxt = xb;
+            for (int l = 0; l <= m->depth; ++l) {
+                const int fi = m->fin[l], fo = m->fout[l];
+                dim3 grid((fo + GBN - 1) / GBN, (B + GBM - 1) / GBM);
+                hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, in, fi, 0, m->pb[e] + m->woff[l], fo, 0, z, fo, B,
+                                   fo, fi, 0.f);
+                HIPCK(hipGetLastError());
+                const bool last = l == m->depth;
+                const size_t nz = (size_t)B * fo;
+                hipLaunchKernelGGL(k_bias_act, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, z,
+                                   m->p[e] + m->boff[l], last ? (uint16_t *)nullptr : nxt, B, fo, last ? 0 : 1);
+                HIPCK(hipGetLastError());
+                std::swap(in, nxt);
+            }
+            hipLaunchKernelGGL(casim::k_na_collect, dim3((B + 255) / 256), dim3(256), 0, st, (const float *)z,
+                               outm + (size_t)e * cap, B);
+            HIPCK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(casim::k_na_reduce, dim3((B + 255) / 256), dim3(256), 0, st, (const float *)outm, m->members,
+                           cap, B, m->ys, m->ym, mean, spread);
+        HIPCK(hipGetLastError());
     });
 }
 

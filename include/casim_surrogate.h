@@ -18,6 +18,7 @@
 #ifndef CASIM_SURROGATE_H
 #define CASIM_SURROGATE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "casim_dyad.h"
@@ -66,6 +67,30 @@ int casim_mlp_gradient(CasimMLP *m, int32_t member, const float *x, const float 
  * concentration units when the target is ln [Ca].  Either output may be NULL;
  * member_out [members x rows] receives every member's raw output. */
 int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, float *member_out);
+
+/*
+ * Ensemble prediction that allocates nothing (no reference counterpart): for
+ * callers that cannot allocate device memory, such as code under HIP graph
+ * capture or a tight serving loop that forbids hipMalloc.  Same result as
+ * casim_mlp_predict (mean and relative spread of exp(prediction)) for up to
+ * CASIM_MLP_NOALLOC_MAX_ROWS rows, but on device arrays: x [rows x n_in]
+ * FP32, mean and spread [rows] (either may be NULL), all device pointers,
+ * work issued on `stream` (a hipStream_t; NULL: the handle's stream) and not
+ * waited for.  The activations live in a fixed workspace of
+ * CASIM_MLP_NOALLOC_BYTES bytes per GPU, held from the first
+ * casim_mlp_create on that GPU until the process ends; calls on one GPU share
+ * it, so calls that overlap on one GPU (two streams, two threads) race.  The
+ * call allocates and frees nothing.  Refused with CASIM_E_RANGE: rows above
+ * the maximum, layers wider than CASIM_MLP_NOALLOC_MAX_WIDTH, more than
+ * CASIM_MLP_NOALLOC_MAX_MEMBERS members.
+ */
+#define CASIM_MLP_NOALLOC_MAX_ROWS 4096
+#define CASIM_MLP_NOALLOC_MAX_WIDTH 256
+#define CASIM_MLP_NOALLOC_MAX_MEMBERS 8
+#define CASIM_MLP_NOALLOC_BYTES \
+    ((size_t)CASIM_MLP_NOALLOC_MAX_ROWS * CASIM_MLP_NOALLOC_MAX_WIDTH * 8 + \
+     (size_t)CASIM_MLP_NOALLOC_MAX_MEMBERS * CASIM_MLP_NOALLOC_MAX_ROWS * 4)
+int casim_mlp_predict_noalloc(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, void *stream);
 
 /*
  * Quasi-stationary release sites driven by the surrogate
