@@ -17,10 +17,11 @@
 //
 // Layout in HBM: box fields [field][site][z][y][x] (x fastest), double,
 // two buffers; the accumulator [exchanged field][site][link].  The box steps
-// every `substeps` dyad steps, so its traffic is a small fraction of the
-// dyad's; its kernels are simple thread-per-node stencils.
+// every `substeps` dyad steps; its step marches in z with the tile's plane in
+// LDS, the rim and efflux kernels are one thread per rim column or link.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -142,116 +143,149 @@ __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring
     }
 }
 
-// One explicit box step.  Block (64, 4): lanes along x, four y-rows.  The
-// 1D grid is a multiple of 8 blocks; block b runs work unit
-// (b % 8) * (grid / 8) + b / 8, so each XCD (blocks are dealt to the 8 XCDs
-// in turn) takes a contiguous range of units -- whole sites, plane after
-// plane in z -- and finds the planes z-1 and z+1 in its own L2.  Ions and
-// bound fields sit in separate arrays whose loops run to the compile-time
-// bounds NI, NB with guards, so every array index is a constant and the
-// per-node values stay in registers.
+// One explicit box step, marching in z.  A workgroup owns one y-tile of one
+// site: rows y0 .. y0+ty-1 of every z-plane plus one halo row on each side.
+// Thread t holds node (x, r) = (t % nx, t / nx) of the tile (box row
+// y0 - 1 + r) for the whole march, with the z-1, z and z+1 values of every
+// field in registers and z+2 loaded while z is computed; the tile's current
+// plane goes through LDS (node-major [node][field], double-buffered, one
+// LDS-only barrier per plane) for the x and y neighbours.  Every stored value
+// is read from HBM once (halo rows a second time, from L2).  Work units are
+// dealt so that consecutive tiles of a site land on the same XCD.  The
+// arithmetic and its order are those of the 7-point step with mirror
+// (no-flux) faces: Laplacian, reactions koff B - kon C (Btot - sum B), SERCA,
+// the z = 0 deposit, the Euler update; far_dirichlet holds the faces x = 0,
+// nx-1, y = 0, ny-1 and z = nz-1.  Ions and bound fields sit in separate
+// arrays whose loops run to the compile-time bounds NI, NB with guards, so
+// every array index is a constant and the values stay in registers.
 template <int NI, int NB>
-__global__ __launch_bounds__(256) void k_box_step(const double *__restrict__ bin, double *__restrict__ bout, BoxArgs a,
-                                                  double dt) {
+__global__ __launch_bounds__(1024) void k_box_step(const double *__restrict__ bin, double *__restrict__ bout, BoxArgs a,
+                                                   double dt, int ty, int ntile) {
 #pragma clang fp contract(off)
+    constexpr int NT = NI + NB;
+    extern __shared__ double blds[];  // [2][tile nodes][NT]
     const int per_xcd = gridDim.x >> 3;
     const int unit = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int nyb = (a.ny + 3) >> 2;
-    const int plane_units = nyb * a.nz;
-    const int s = unit / plane_units;
-    if (s >= a.S) return;
-    const int r = unit - s * plane_units;
-    const int z = r / nyb;
-    const int x = threadIdx.x, y = (r - z * nyb) * 4 + threadIdx.y;
-    if (x >= a.nx || y >= a.ny) return;
-    const int n = x + a.nx * (y + a.ny * z);
+    const int s = unit / ntile;
+    if (s >= a.S) return;  // whole workgroup
+    const int y0 = (unit - s * ntile) * ty;
+    const int TN = (ty + 2) * a.nx;
+    const int t = threadIdx.x;
+    const int x = t % a.nx, r = t / a.nx;
+    const int y = y0 - 1 + r;
+    const bool live = t < TN && y >= 0 && y < a.ny;
+    const bool comp = live && r >= 1 && r <= ty;
+    const bool held = a.far_dirichlet && (x == 0 || x == a.nx - 1 || y == 0 || y == a.ny - 1);
+    // LDS offsets (in nodes) of the x and y neighbours, mirrored on the faces
+    const int oxl = x > 0 ? -1 : 1, oxr = x < a.nx - 1 ? 1 : -1;
+    const int oyl = y > 0 ? -a.nx : a.nx, oyr = y < a.ny - 1 ? a.nx : -a.nx;
     const size_t per = (size_t)a.S * a.NB;
+    const size_t plane = (size_t)a.nx * a.ny;
     const double *ib = bin + (size_t)s * a.NB;
     double *ob = bout + (size_t)s * a.NB;
-    if (a.far_dirichlet && (x == 0 || x == a.nx - 1 || y == 0 || y == a.ny - 1 || z == a.nz - 1)) {
-        for (int f = 0; f < a.nf; ++f) ob[f * per + n] = ib[f * per + n];
-        return;
-    }
-    // mirror (no-flux) neighbours on the faces
-    const int sy = a.nx, sz = a.nx * a.ny;
-    const int xl = x > 0 ? n - 1 : n + 1, xr = x < a.nx - 1 ? n + 1 : n - 1;
-    const int yl = y > 0 ? n - sy : n + sy, yr = y < a.ny - 1 ? n + sy : n - sy;
-    const int zl = z > 0 ? n - sz : n + sz, zr = z < a.nz - 1 ? n + sz : n - sz;
-    // all loads first (7 per field), so that every field's memory latency overlaps
-    constexpr int NT = NI + NB;
-    double q[NT][7];
+    const int nxy = x + a.nx * (live ? y : 0);
+    auto on = [&](int f) { return f < NI ? f < a.ni : f - NI < a.nbb; };
+    auto fld = [&](int f) { return f < NI ? f : a.ni + f - NI; };
+    // Q[k]: plane z - 1 + k of every field (k = 3: the prefetch of z + 2)
+    double Q[4][NT];
+    auto load = [&](double (&D)[NT], int z) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const bool on = t < NI ? t < a.ni : t - NI < a.nbb;
-        const double *u = ib + (t < NI ? t : a.ni + t - NI) * per;
-        const int o[7] = {n, xl, xr, yl, yr, zl, zr};
-#pragma unroll
-        for (int k = 0; k < 7; ++k) q[t][k] = on ? u[o[k]] : 0.0;
-    }
-    auto diffuse = [&](int t, int f, double &vo, double &eo) {
-        const double vc = q[t][0];
-        vo = vc;
-        const double lap = (q[t][2] - 2 * vc + q[t][1]) * a.ih2 + (q[t][4] - 2 * vc + q[t][3]) * a.ih2 +
-                           (q[t][6] - 2 * vc + q[t][5]) * a.ih2;
-        eo = a.D[f] * lap;
+        for (int f = 0; f < NT; ++f) D[f] = (live && on(f)) ? ib[fld(f) * per + z * plane + nxy] : 0.0;
     };
-    double vi[NI], ei[NI], di[NI], vb[NB], eb[NB], db[NB];
+    load(Q[1], 0);
+    if (a.nz > 1) load(Q[2], 1);
+    for (int z = 0; z < a.nz; ++z) {
+        if (z + 2 < a.nz) load(Q[3], z + 2);
+        double *L = blds + (size_t)(z & 1) * TN * NT;
+        if (t < TN)
 #pragma unroll
-    for (int j = 0; j < NI; ++j) {
-        vi[j] = ei[j] = di[j] = 0.0;
-        if (j < a.ni) diffuse(j, j, vi[j], ei[j]);
-    }
+            for (int f = 0; f < NT; ++f) L[t * NT + f] = Q[1][f];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        if (comp) {
+            const size_t n = (size_t)nxy + z * plane;
+            if (held || (a.far_dirichlet && z == a.nz - 1)) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        vb[b] = eb[b] = db[b] = 0.0;
-        if (b < a.nbb) diffuse(NI + b, a.ni + b, vb[b], eb[b]);
-    }
+                for (int f = 0; f < NT; ++f)
+                    if (on(f)) ob[fld(f) * per + n] = Q[1][f];
+            } else {
+                const double *Lc = L + (size_t)t * NT;
+                auto diffuse = [&](int f, int fd, double &vo, double &eo) {
+                    const double vc = Q[1][f];
+                    vo = vc;
+                    const double zl = z > 0 ? Q[0][f] : Q[2][f];
+                    const double zr = z < a.nz - 1 ? Q[2][f] : Q[0][f];
+                    const double lap = (Lc[oxr * NT + f] - 2 * vc + Lc[oxl * NT + f]) * a.ih2 +
+                                       (Lc[oyr * NT + f] - 2 * vc + Lc[oyl * NT + f]) * a.ih2 + (zr - 2 * vc + zl) * a.ih2;
+                    eo = a.D[fd] * lap;
+                };
+                double vi[NI], ei[NI], di[NI], vb[NB], eb[NB], db[NB];
 #pragma unroll
-    for (int j = 0; j < NI; ++j) {
-        if (j >= a.ni) continue;
-        double R = 0.0;
+                for (int j = 0; j < NI; ++j) {
+                    vi[j] = ei[j] = di[j] = 0.0;
+                    if (j < a.ni) diffuse(j, j, vi[j], ei[j]);
+                }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (b >= a.nbb || a.bion[b] != j) continue;
-            // free buffer: total minus its bound fields, in bound-field order
-            double fr = a.total[a.bbuf[b]];
+                for (int b = 0; b < NB; ++b) {
+                    vb[b] = eb[b] = db[b] = 0.0;
+                    if (b < a.nbb) diffuse(NI + b, a.ni + b, vb[b], eb[b]);
+                }
 #pragma unroll
-            for (int c = 0; c < NB; ++c)
-                if (c < a.nbb && a.bbuf[c] == a.bbuf[b]) fr -= vb[c];
-            const double R1 = a.koff[b] * vb[b] - a.kon[b] * vi[j] * fr;
-            eb[b] += -R1;
-            R += R1;
-        }
-        ei[j] += R;
-        if (a.jmax[j] != 0.0) {
-            const double c2 = vi[j] * vi[j];
-            ei[j] -= a.jmax[j] * (c2 / (c2 + a.K2[j]) - a.base[j]);
-        }
-    }
-    if (z == 0) {
-        const int row = a.dep_row[n];
-        if (row >= 0) {
-            const int k0 = a.dep_ptr[row], k1 = a.dep_ptr[row + 1];
-            for (int e = 0; e < a.ne; ++e) {
-                const double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink;
-                double sum = 0.0;
-                for (int k = k0; k < k1; ++k) sum += a.dep_w[k] * acc[a.dep_link[k]];
-                const int f = a.ex_box[e];
+                for (int j = 0; j < NI; ++j) {
+                    if (j >= a.ni) continue;
+                    double R = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        if (b >= a.nbb || a.bion[b] != j) continue;
+                        // free buffer: total minus its bound fields, in bound-field order
+                        double fr = a.total[a.bbuf[b]];
+#pragma unroll
+                        for (int c = 0; c < NB; ++c)
+                            if (c < a.nbb && a.bbuf[c] == a.bbuf[b]) fr -= vb[c];
+                        const double R1 = a.koff[b] * vb[b] - a.kon[b] * vi[j] * fr;
+                        eb[b] += -R1;
+                        R += R1;
+                    }
+                    ei[j] += R;
+                    if (a.jmax[j] != 0.0) {
+                        const double c2 = vi[j] * vi[j];
+                        ei[j] -= a.jmax[j] * (c2 / (c2 + a.K2[j]) - a.base[j]);
+                    }
+                }
+                if (z == 0) {
+                    const int row = a.dep_row[nxy];
+                    if (row >= 0) {
+                        const int k0 = a.dep_ptr[row], k1 = a.dep_ptr[row + 1];
+                        for (int e = 0; e < a.ne; ++e) {
+                            const double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink;
+                            double sum = 0.0;
+                            for (int k = k0; k < k1; ++k) sum += a.dep_w[k] * acc[a.dep_link[k]];
+                            const int f = a.ex_box[e];
+#pragma unroll
+                            for (int j = 0; j < NI; ++j)
+                                if (f == j) di[j] = sum;
+#pragma unroll
+                            for (int b = 0; b < NB; ++b)
+                                if (f == a.ni + b) db[b] = sum;
+                        }
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < NI; ++j)
-                    if (f == j) di[j] = sum;
+                    if (j < a.ni) ob[j * per + n] = vi[j] + ei[j] * dt + di[j] * a.inv_face_volume;
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
-                    if (f == a.ni + b) db[b] = sum;
+                    if (b < a.nbb) ob[(a.ni + b) * per + n] = vb[b] + eb[b] * dt + db[b] * a.inv_face_volume;
             }
         }
+#pragma unroll
+        for (int f = 0; f < NT; ++f) {
+            Q[0][f] = Q[1][f];
+            Q[1][f] = Q[2][f];
+            Q[2][f] = Q[3][f];
+        }
     }
-#pragma unroll
-    for (int j = 0; j < NI; ++j)
-        if (j < a.ni) ob[j * per + n] = vi[j] + ei[j] * dt + di[j] * a.inv_face_volume;
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-        if (b < a.nbb) ob[(a.ni + b) * per + n] = vb[b] + eb[b] * dt + db[b] * a.inv_face_volume;
 }
 
 // ---- host side -------------------------------------------------------------
@@ -296,14 +330,26 @@ void *box_ring(const CasimDyad *d) { return d->box ? d->box->ring : nullptr; }
 
 void box_step(CasimDyad *d, double dt) {
     CytoBox *b = d->box;
-    const long units = (long)((b->a.ny + 3) / 4) * b->a.nz * d->P.S;
+    const BoxArgs &a = b->a;
+    const bool small = a.ni == 1 && a.nbb <= 3;
+    const int nt = small ? 4 : MAXI + MAXB;  // fields per LDS node
+    // y-tiles of ty rows (+ 2 halo rows): about 512 nodes, and two LDS planes
+    // within 64 KB; nx <= 64 (attach)
+    const int rows = std::min(512 / a.nx, (int)(65536 / (2 * sizeof(double) * nt * a.nx)));
+    const int ty = std::min(a.ny, std::max(1, rows - 2));
+    const int ntile = (a.ny + ty - 1) / ty;
+    const int threads = ((ty + 2) * a.nx + 63) / 64 * 64;
+    const size_t lds = 2 * sizeof(double) * nt * (ty + 2) * a.nx;
+    const long units = (long)ntile * d->P.S;
     REQUIRE(units < (1L << 31) - 8, CASIM_E_INVALID, "cytosol box batch too large for one launch");
-    dim3 grid((unsigned)((units + 7) / 8 * 8)), block(64, 4);
-    if (b->a.ni == 1 && b->a.nbb <= 3)
-        hipLaunchKernelGGL((k_box_step<1, 3>), grid, block, 0, d->stream, b->b[b->cur], b->b[b->cur ^ 1], b->a, dt);
+    REQUIRE(threads <= 1024 && lds <= 65536, CASIM_E_INVALID, "cytosol box tile does not fit a workgroup");
+    dim3 grid((unsigned)((units + 7) / 8 * 8));
+    if (small)
+        hipLaunchKernelGGL((k_box_step<1, 3>), grid, dim3(threads), lds, d->stream, b->b[b->cur], b->b[b->cur ^ 1], a,
+                           dt, ty, ntile);
     else
-        hipLaunchKernelGGL((k_box_step<MAXI, MAXB>), grid, block, 0, d->stream, b->b[b->cur], b->b[b->cur ^ 1],
-                           b->a, dt);
+        hipLaunchKernelGGL((k_box_step<MAXI, MAXB>), grid, dim3(threads), lds, d->stream, b->b[b->cur],
+                           b->b[b->cur ^ 1], a, dt, ty, ntile);
     HIPCK(hipGetLastError());
     b->cur ^= 1;
 }
@@ -381,7 +427,7 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
         REQUIRE(!d->box, CASIM_E_STATE, "a cytosol box is already attached");
         REQUIRE(!(d->Q.on && d->Q.cyto), CASIM_E_STATE, "a cytosol pool is attached");
         REQUIRE(c->nx >= 4 && c->ny >= 4 && c->nz >= 2, CASIM_E_INVALID, "box needs nx, ny >= 4 and nz >= 2");
-        REQUIRE(c->nx <= 64 && c->nz <= 65535, CASIM_E_INVALID, "box needs nx <= 64 (one wave per x-row)");
+        REQUIRE(c->nx <= 64 && c->nz <= 65535, CASIM_E_INVALID, "box needs nx <= 64 (x-rows of the box step's tiles)");
         REQUIRE((int64_t)c->nx * c->ny * c->nz < (1LL << 30), CASIM_E_INVALID, "box too large");
         REQUIRE(c->h > 0 && std::isfinite(c->h), CASIM_E_INVALID, "box spacing must be > 0");
         REQUIRE(c->substeps >= 1, CASIM_E_INVALID, "substeps must be >= 1");
