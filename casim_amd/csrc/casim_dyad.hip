@@ -436,6 +436,8 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         d->d_jsr = dalloc<double>((size_t)P.ni * P.S * std::max(1, d->M));
         d->d_sJ = dalloc<double>((size_t)P.ni * std::max(1, d->nch));
         d->d_snap = dalloc<double>((size_t)(P.n_grid + P.n_face) * std::max(1, d->nch));
+        d->d_pg = dalloc<double>((size_t)P.ni * std::max(1, d->nch));
+        d->d_pjv = dalloc<double>((size_t)P.ni * std::max(1, d->nch));
         d->d_totals = dalloc<double>((size_t)P.ni * P.S);
         d->d_rim_ions = dalloc<double>((size_t)P.ni * P.S);
         d->d_rim_bound = dalloc<double>((size_t)std::max(1, P.nb) * P.S);
@@ -469,6 +471,8 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         Ch.jsr = d->d_jsr;
         Ch.sJ = d->d_sJ;
         Ch.snap = d->d_snap;
+        Ch.pg = d->d_pg;
+        Ch.pjv = d->d_pjv;
         Ch.totals = d->d_totals;
         Ch.stats = d->d_stats;
         Ch.trace = nullptr;
@@ -494,7 +498,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     casim::box_free(d);
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
                     d->d_sperm, d->d_state, d->d_chans, d->d_rng, d->d_rng0, d->d_jump, d->d_mouths, d->d_jsr, d->d_sJ,
-                    d->d_snap, d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats, d->Ch.trace};
+                    d->d_snap, d->d_pg, d->d_pjv, d->d_totals, d->d_rim_ions, d->d_rim_bound, d->d_eff, d->d_stats, d->Ch.trace};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : d->pool_buffers)

@@ -58,6 +58,7 @@ struct BoxArgs {
     const int *link_col;    // [nlink] rim column
     const int *link_ring;   // [nlink] ring index of the interior end (ring_index)
     int RN;                 // ring values per site and grid field (ring nodes x dyad z nodes)
+    int ngrid;              // dyad grid fields (ring halves hold every one)
     void *rimc;             // [ne][S][ncol] the rim values k_box_rim wrote (dyad precision)
     const double *link_fac; // [nlink] A / d per unit z weight
     const double *zw;       // [dnz] z weights (1 in 2D; 1/2 on the two z-planes in 3D)
@@ -73,7 +74,8 @@ struct CytoBox {
     int substeps = 1;
     int cur = 0;
     double *b[2] = {nullptr, nullptr};
-    void *ring = nullptr;      // [grid field][site][RN] step-k+1 values written by k_sweep2
+    void *ring = nullptr;      // two halves, each [grid field][site][RN], written by k_sweep2: the
+                               // pass's step-k+1 values, then its step-k (input) values
     std::vector<double> init;  // [nf] rest values
     double *d_init = nullptr;  // the same on the device
     std::vector<void *> owned;
@@ -109,9 +111,11 @@ __global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs 
 }
 
 // Rim-link effluxes of the dyad state g, times dt, into the accumulator:
-// one thread per (link, site).  With ring non-null the thread then adds the
-// next step's effluxes too, from the ring values a two-step pass left (the
-// rim is the same for both steps): acc + dt e(k), then + dt e(k+1), the
+// one thread per (link, site).  After a two-step pass (ring non-null) both
+// steps come from the ring the pass wrote next to the rim (contiguous, so no
+// strided reads of the interior ends of the y-rim links): step k from its
+// second half (the pass's input), step k+1 from its first half; the rim is
+// the same for both steps.  The sums are acc + dt e(k), then + dt e(k+1), the
 // order of two separate calls.
 template <typename T>
 __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring, BoxArgs a, double dt, int first) {
@@ -121,12 +125,13 @@ __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring
     if (l >= a.nlink) return;
     const size_t in = (size_t)s * a.dNS + a.link_in[l];
     const size_t rin = (size_t)s * a.RN + (size_t)a.link_ring[l] * a.dnz;
+    const size_t half = (size_t)a.ngrid * a.S * a.RN;
     const double fac = a.link_fac[l];
     const T *rimc = (const T *)a.rimc;
     for (int e = 0; e < a.ne; ++e) {
         // the rim node's value (every z of its column): what k_box_rim wrote there
         const double ur = (double)rimc[((size_t)e * a.S + s) * a.ncol + a.link_col[l]];
-        const T *ui = g + a.ex_grid[e] * a.dfs + in;
+        const T *ui = ring ? ring + half + (size_t)a.ex_grid[e] * a.S * a.RN + rin : g + a.ex_grid[e] * a.dfs + in;
         double t = 0.0;
         for (int z = 0; z < a.dnz; ++z) t += a.zw[z] * ((double)ui[z] - ur);
         double *acc = a.acc + ((size_t)e * a.S + s) * a.nlink + l;
@@ -321,8 +326,8 @@ void efflux_launch(CasimDyad *d, const void *g, const void *ring, bool first, do
 }  // namespace
 
 void box_efflux(CasimDyad *d, bool first, double dt) { efflux_launch(d, d->g[d->cur], nullptr, first, dt); }
-// after a two-step pass: the pass's input buffer (now g[cur ^ 1]) still holds
-// step k, and the ring holds step k+1 next to the rim
+// after a two-step pass: the ring holds steps k+1 and k next to the rim (the
+// pass's input buffer, now g[cur ^ 1], holds step k everywhere)
 void box_efflux_pair(CasimDyad *d, bool first, double dt) {
     efflux_launch(d, d->g[d->cur ^ 1], d->box->ring, first, dt);
 }
@@ -609,7 +614,8 @@ int casim_dyad_attach_cytosol_box(CasimDyad *d, const CasimCytosolBoxConfig *c) 
             a.link_col = upload(b, link_col, d->stream);
             a.link_ring = upload(b, link_ring, d->stream);
             a.RN = ring_nodes(nx, ny) * a.dnz;
-            HIPCK(hipMalloc(&b->ring, d->tsize * (size_t)P.n_grid * P.S * a.RN));
+            a.ngrid = P.n_grid;
+            HIPCK(hipMalloc(&b->ring, 2 * d->tsize * (size_t)P.n_grid * P.S * a.RN));
             b->owned.push_back(b->ring);
             HIPCK(hipMalloc(&a.rimc, d->tsize * (size_t)a.ne * P.S * a.ncol));
             b->owned.push_back(a.rimc);

@@ -80,7 +80,7 @@ struct CasimDyad {
     int *d_off = nullptr, *d_node = nullptr, *d_type = nullptr, *d_perm = nullptr, *d_snode = nullptr, *d_sperm = nullptr,
         *d_state = nullptr, *d_chans = nullptr;
     uint32_t *d_rng = nullptr, *d_rng0 = nullptr, *d_jump = nullptr;
-    double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_snap = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
+    double *d_mouths = nullptr, *d_jsr = nullptr, *d_sJ = nullptr, *d_snap = nullptr, *d_pg = nullptr, *d_pjv = nullptr, *d_totals = nullptr, *d_rim_ions = nullptr,
            *d_rim_bound = nullptr, *d_eff = nullptr, *d_stats = nullptr;
     bool have_update = false;
     bool start_equilibrium = false;

@@ -55,6 +55,8 @@ struct Channels {
     double *mouths;     // [K*S*M]  K = ni + nb + nu
     double *snap;       // [(n_grid + n_face) * nch] step-k+1 values of the mouth nodes, node-sorted,
                         // stored by k_sweep2 (face values 0 off the two z-faces)
+    double *pg;         // [ni * nch] node-sorted: state * conductance after the last gating
+    double *pjv;        // [ni * nch] node-sorted: the jSR each channel sees (constant jSR, no pools)
     const double *jsr;  // [M*S*ni]
     double *sJ;         // [ni*nch] currents in node-sorted order
     double *totals;     // [ni*S]
@@ -429,20 +431,20 @@ template <int LY> __device__ __forceinline__ double ly_total(const Params &p, in
 // (every grid field) and, at face nodes, the new face values to Fo.
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int LI_ = NI ? NI : MAXI>
 __device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], const T (&C0)[NG],
-                                            const T (&Cp)[NG], const T *L, int c, int P, int zl, int zr, int z,
+                                            const T (&Cp)[NG], const T *Lc, int sy, int zl, int zr, int z,
                                             bool face, const T (&Fv)[NF], const T (&J)[LI_], T dt, T ihx2, T ihy2, T ihz2,
                                             T (&G)[NG], T (&Fo)[NF]) {
     constexpr int LI = NI ? NI : MAXI;
     constexpr int LB = NB ? NB : MAXB;
     T Cv[LI], eC[LI], Bv[LB], eB[LB];
+    const T *Lyl = Lc - sy, *Lyr = Lc + sy, *Lzl = Lc + zl, *Lzr = Lc + zr;
 #pragma unroll
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
         const T vc = C0[j];
         Cv[j] = vc;
-        const T *u = L + j * P + c;
-        T lap = (Cp[j] - 2 * vc + Cm[j]) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
-        if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+        T lap = (Cp[j] - 2 * vc + Cm[j]) * ihx2 + (Lyr[j] - 2 * vc + Lyl[j]) * ihy2;
+        if (DIM == 3) lap = lap + (Lzr[j] - 2 * vc + Lzl[j]) * ihz2;
         eC[j] = (T)p.Dion[j] * lap;
     }
 #pragma unroll
@@ -467,9 +469,8 @@ __device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], 
                 }
             Bv[b] = vc;
             if (ly_mobile<LY>(p, b)) {
-                const T *u = L + g * P + c;
-                T lap = (cxr - 2 * vc + cxl) * ihx2 + (u[p.nz] - 2 * vc + u[-p.nz]) * ihy2;
-                if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
+                T lap = (cxr - 2 * vc + cxl) * ihx2 + (Lyr[g] - 2 * vc + Lyl[g]) * ihy2;
+                if (DIM == 3) lap = lap + (Lzr[g] - 2 * vc + Lzl[g]) * ihz2;
                 eB[b] = (T)p.Db[b] * lap;
             } else {
                 eB[b] = 0;
@@ -556,7 +557,7 @@ struct PlaneSlots {
 // Index of interior node (x, y) in the ring of nodes next to the rim (x = 1,
 // x = nx-2, y = 1, y = ny-2), or -1: x = 1 first, then x = nx-2, then the
 // y = 1 and y = ny-2 rows without their ends.  The cytosol box reads the
-// step-k+1 values that k_sweep2 leaves there.
+// step-k+1 and step-k values that k_sweep2 leaves there.
 __host__ __device__ __forceinline__ int ring_index(int x, int y, int nx, int ny) {
     if (x == 1) return y - 1;
     if (x == nx - 2) return (ny - 2) + y - 1;
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && ps.act[k]) L[g * P + ps.q[k]] = R[1][k][g];
+                if (g < ng && ps.act[k]) L[ps.q[k] * ng + g] = R[1][k][g];
         lds_barrier();
 
         // channel currents on this plane (the last channel on a node wins)
@@ -687,7 +688,8 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
             const int c = ps.q[k];
             const int z = DIM == 3 ? c % p.nz : 0;
             T G[NG], Fo[NF];
-            node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], L, c, P, ps.zl[k], ps.zr[k], z,
+            node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], L + c * ng, p.nz * ng,
+                                                ps.zl[k] * ng, ps.zr[k] * ng, z,
                                                 ps.fz[k] >= 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
             const size_t o = (size_t)x * P + c;
 #pragma unroll
@@ -736,12 +738,9 @@ template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL,
 __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
                                                 const T *__restrict__ fin, T *__restrict__ fout,
                                                 const int *__restrict__ ch_off, const int *__restrict__ snode,
-                                                const int *__restrict__ sperm, const double *__restrict__ sJ,
-                                                const int *__restrict__ cstate, const int *__restrict__ ctype,
-                                                const double *__restrict__ jsr, const double *__restrict__ jnew,
-                                                double *__restrict__ mouths, T *__restrict__ ring,
-                                                const Channels a, Params p, double dt_, int chunk, int nchunks,
-                                                int units) {
+                                                const double *__restrict__ sJ, const double *__restrict__ jnew,
+                                                T *__restrict__ ring, const Channels a, Params p, double dt_,
+                                                int chunk, int nchunks, int units) {
     constexpr int LI = NI ? NI : MAXI;
     constexpr int LB = NB ? NB : MAXB;
     constexpr int NG = NGT ? NGT : LI + LB;
@@ -812,7 +811,6 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
     const int chi = ch_off[s + 1];
     int curA = first_channel_at(snode, ch_off[s], chi, xa0 * P);
     int curB = first_channel_at(snode, ch_off[s], chi, x0 * P);
-    const int off = ch_off[s];
 
     for (int x = xa0; x <= x1; ++x) {
 #pragma unroll
@@ -830,8 +828,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
 #pragma unroll
             for (int g = 0; g < NG; ++g)
                 if (g < ng && (fl[k] & 1)) {
-                    if (interiorA) La[g * P + ps.q[k]] = R[1][k][g];
-                    if (doB) Lb[g * P + ps.q[k]] = A[1][k][g];
+                    if (interiorA) La[ps.q[k] * ng + g] = R[1][k][g];
+                    if (doB) Lb[ps.q[k] * ng + g] = A[1][k][g];
                 }
         lds_barrier();
 
@@ -860,7 +858,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                 if (interiorA && (fl[k] & 2)) {
                     const int c = ps.q[k];
                     const int z = DIM == 3 ? c % p.nz : 0;
-                    node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], La, c, P, ps.zl[k], ps.zr[k],
+                    node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], La + c * ng, p.nz * ng,
+                                                        ps.zl[k] * ng, ps.zr[k] * ng,
                                                         z, (fl[k] & 4) != 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2,
                                                         A[2][k], FA[1][k]);
                 } else {
@@ -870,7 +869,7 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                     for (int f = 0; f < NF; ++f) FA[1][k][f] = F[0][k][f];
                 }
             }
-            // step-k+1 values of the nodes next to the rim, for a cytosol box
+            // step-k+1 and step-k values of the nodes next to the rim, for a cytosol box
             if (RING && interiorA) {
                 const int nzd = DIM == 3 ? p.nz : 1;
                 const size_t RN = (size_t)ring_nodes(p.nx, p.ny) * nzd;
@@ -881,9 +880,13 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                     if (ri < 0) continue;
                     const int z = DIM == 3 ? ps.q[k] % p.nz : 0;
                     T *rp = ring + (size_t)s * RN + (size_t)ri * nzd + z;
+                    T *rp0 = rp + (size_t)ng * p.S * RN;  // second half: the step-k values
 #pragma unroll
                     for (int g = 0; g < NG; ++g)
-                        if (g < ng) rp[g * (size_t)p.S * RN] = A[2][k][g];
+                        if (g < ng) {
+                            rp[g * (size_t)p.S * RN] = A[2][k][g];
+                            rp0[g * (size_t)p.S * RN] = R[1][k][g];
+                        }
                 }
             }
         }
@@ -901,23 +904,20 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                 const int nd = snode[curB];
                 if (nd >= pend) break;
                 const int qq = nd - xb * P;
-                const int i = sperm[curB];
 #pragma unroll
                 for (int k = 0; k < SL; ++k)
                     if (ps.q[k] == qq) {
-                        // the channel kernel's current (k_channels) from the step-k+1
-                        // mouth value; the node's step-k+1 values go to the snapshot
-                        // the next channel call reads (CH_FROM_STORED)
-                        const int st = cstate[off + i];
-                        const int typ = ctype[off + i];
+                        // the channel kernel's current, state * g * (jSR - mouth), from
+                        // the step-k+1 mouth value, with state * g as the channel call
+                        // before the pass left it; the node's step-k+1 values go to the
+                        // snapshot the next channel call reads (CH_FROM_STORED)
 #pragma unroll
                         for (int j = 0; j < LI; ++j) {
                             if (!NI && j >= p.ni) break;
                             const double ion = (double)A[1][k][j];
                             // jSR of the step: the device pool's value when pools are on
-                            const double jv = jnew ? jnew[s + j * p.S] : jsr[i + (size_t)a.M * (s + (size_t)j * p.S)];
-                            const double cond = a.g[typ * MAXI + j] * (jv - ion);
-                            J[k][j] = (T)((double)st * cond);
+                            const double jv = jnew ? jnew[s + j * p.S] : a.pjv[(size_t)j * a.nch + curB];
+                            J[k][j] = (T)(a.pg[(size_t)j * a.nch + curB] * (jv - ion));
                         }
                         double *sn = a.snap + curB;
 #pragma unroll
@@ -935,7 +935,8 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                 const int c = ps.q[k];
                 const int z = DIM == 3 ? c % p.nz : 0;
                 T G[NG], Fo[NF];
-                node_update<T, DIM, NI, NB, NG, NF, LY>(p, A[0][k], A[1][k], A[2][k], Lb, c, P, ps.zl[k], ps.zr[k], z,
+                node_update<T, DIM, NI, NB, NG, NF, LY>(p, A[0][k], A[1][k], A[2][k], Lb + c * ng, p.nz * ng,
+                                                    ps.zl[k] * ng, ps.zr[k] * ng, z,
                                                     (fl[k] & 4) != 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
                 const size_t o = (size_t)xb * P + c;
 #pragma unroll
@@ -1115,6 +1116,7 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
                     double cond = a.g[typ * MAXI + k] * (jv - ion[k]);
                     J = (double)st * cond;
                     a.sJ[(size_t)k * a.nch + off + sp] = J;
+                    if (!q.on) a.pjv[(size_t)k * a.nch + off + sp] = jv;
                 }
                 Jl[k][lane] = J;
             }
@@ -1153,7 +1155,11 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
                 pos += (c0 + jl - first) + 2;
                 first = c0 + jl + 1;
             }
-            if (valid) a.state[off + i] = st;
+            if (valid) {
+                a.state[off + i] = st;
+                const int sp = a.perm[off + i];
+                for (int k = 0; k < p.ni; ++k) a.pg[(size_t)k * a.nch + off + sp] = (double)st * a.g[typ * MAXI + k];
+            }
             nopen += __popcll(__ballot(valid && st == 1));
         }
     }

@@ -69,9 +69,9 @@ void sweep2_ring(CasimDyad *d, double dt) {
     hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL, RING>), dim3(grid), dim3(d->sweep_threads), lds,
                        d->stream,
                        (const T *)d->g[d->cur], (T *)d->g[d->cur ^ 1], (const T *)d->f[d->cur],
-                       (T *)d->f[d->cur ^ 1], d->d_off, d->d_snode, d->d_sperm, d->d_sJ, d->d_state, d->d_type,
-                       d->d_jsr, (const double *)(d->Q.on ? d->Q.jnew : nullptr), d->d_mouths,
-                       (T *)box_ring(d), d->Ch, d->P, dt, chunk, nchunks, units);
+                       (T *)d->f[d->cur ^ 1], d->d_off, d->d_snode, d->d_sJ,
+                       (const double *)(d->Q.on ? d->Q.jnew : nullptr), (T *)box_ring(d), d->Ch, d->P, dt, chunk,
+                       nchunks, units);
 }
 
 // The ring store (step-k+1 values next to the rim, for a cytosol box) is a
