@@ -62,9 +62,15 @@ def apply(dev, spec):
 
 
 def tune(dev, spec, dt, steps=20, warmup=4, save=True, log=None):
-    """Time every candidate on `dev` (a DyadHIP after its first Update)."""
+    """Time every candidate on `dev` (a DyadHIP after its first Update).
+    The prefetch depth belongs to the one-step sweep only, so with two steps
+    per pass each (planes, threads) shape is timed once."""
     res = []
-    for planes, th, pf in candidates(spec):
+    cands = candidates(spec)
+    if dev.StepsPerPass() == 2:
+        seen = set()
+        cands = [c for c in cands if (c[0], c[1]) not in seen and not seen.add((c[0], c[1]))]
+    for planes, th, pf in cands:
         try:
             dev.SetTuning(planes, th, pf)
         except RuntimeError:

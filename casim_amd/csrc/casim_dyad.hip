@@ -144,12 +144,21 @@ void CasimDyad::launch_pair(double dt) {
 }
 
 void CasimDyad::launch_channels(int mode, double dt) {
-    if (precision == CASIM_FP64)
-        hipLaunchKernelGGL((k_channels<double>), dim3(P.S), dim3(64), 0, stream, (const double *)g[cur],
-                           (const double *)f[cur], Ch, P, Q, mode, dt);
-    else
-        hipLaunchKernelGGL((k_channels<float>), dim3(P.S), dim3(64), 0, stream, (const float *)g[cur],
-                           (const float *)f[cur], Ch, P, Q, mode, dt);
+    if (precision == CASIM_FP64) {
+        if (chan_small)
+            hipLaunchKernelGGL((k_channels<double, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), dim3(P.S), dim3(64), 0, stream,
+                               (const double *)g[cur], (const double *)f[cur], Ch, P, Q, mode, dt);
+        else
+            hipLaunchKernelGGL((k_channels<double, 0, 0, 0, 0>), dim3(P.S), dim3(64), 0, stream, (const double *)g[cur],
+                               (const double *)f[cur], Ch, P, Q, mode, dt);
+    } else {
+        if (chan_small)
+            hipLaunchKernelGGL((k_channels<float, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), dim3(P.S), dim3(64), 0, stream,
+                               (const float *)g[cur], (const float *)f[cur], Ch, P, Q, mode, dt);
+        else
+            hipLaunchKernelGGL((k_channels<float, 0, 0, 0, 0>), dim3(P.S), dim3(64), 0, stream, (const float *)g[cur],
+                               (const float *)f[cur], Ch, P, Q, mode, dt);
+    }
     HIPCK(hipGetLastError());
 }
 
@@ -386,6 +395,11 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
             for (int q = 0; q < NPAR; ++q) Ch.par[t * NPAR + q] = c->type_params[t * CASIM_MAX_MODEL_PARAMS + q];
             for (int k = 0; k < MAXI; ++k) Ch.g[t * MAXI + k] = k < P.ni ? c->type_conductance[t * P.ni + k] : 0.0;
         }
+        // the channel kernel's compile-time form: every type SheepCannell with one
+        // ion, three bound fields and three buffers, or the general one
+        bool sheep = true;
+        for (int t = 0; t < c->n_channel_types; ++t) sheep = sheep && Ch.model[t] == CASIM_MODEL_SHEEP_CANNELL;
+        d->chan_small = sheep && P.ni == 1 && P.nb == 3 && P.nu == 3;
         Ch.M = d->M;
         Ch.nch = d->nch;
         REQUIRE(c->calcium_index >= -1 && c->calcium_index < P.ni && c->magnesium_index >= -1 &&

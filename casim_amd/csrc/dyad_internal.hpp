@@ -90,6 +90,7 @@ struct CasimDyad {
     int sweep_prefetch = 1;   // planes prefetched beyond x+1
     bool use_sweep = true;
     int steps_per_pass = 2;   // 2: pairs of steps in one k_sweep2 pass where pair_ok()
+    bool chan_small = false;  // the channel kernel's SheepCannell / one-ion instantiation applies
     casim::CytoBox *box = nullptr;  // cytosol box (null when not attached)
     // captured K-step graphs, keyed by (k_steps, dt, state buffers in use:
     // bit 0 the dyad's, bit 1 the box's), and the buffer flips each makes

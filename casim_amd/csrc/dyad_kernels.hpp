@@ -1014,6 +1014,13 @@ __device__ __forceinline__ double channel_rate(int model, const double *q, int s
     return 0.0;
 }
 
+// The rate with the model fixed at compile time (MODEL > 0) or read per
+// channel type (MODEL = 0).
+template <int MODEL>
+__device__ __forceinline__ double channel_rate_m(int model, const double *q, int state, double ca, double mg) {
+    return channel_rate(MODEL ? MODEL : model, q, state, ca, mg);
+}
+
 // Draw number p (0-based) after state S: jump 2p steps, then one U01d draw.
 __device__ __forceinline__ double draw_at(const uint32_t *__restrict__ jump, const uint32_t S[6], int p) {
     const uint32_t *J = jump + (size_t)p * 18;
@@ -1044,10 +1051,12 @@ __device__ __forceinline__ double draw_at(const uint32_t *__restrict__ jump, con
 // (draws so far + lanes before them) by jumping the step-start state with
 // A^(2p); a ballot finds the first toggling channel, positions after it are
 // re-evaluated with the extra draw accounted for, until a ballot is empty.
-template <typename T>
+template <typename T, int MODEL, int NI_, int NB_, int NU_>
 __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const T *__restrict__ f, Channels a,
                                                  Params p, Pools q, int mode, double dt) {
-    __shared__ double Jl[MAXI][64];
+    // compile-time species counts where the layout is known (0: up to the maxima, guarded)
+    constexpr int LI = NI_ ? NI_ : MAXI, LB = NB_ ? NB_ : MAXB, LU = NU_ ? NU_ : MAXU;
+    __shared__ double Jl[LI][64];
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
     const int off = a.off[s];
@@ -1058,9 +1067,9 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
     for (int i = 0; i < 6; ++i) S6[i] = a.rng[s * 6 + i];
     int pos = 0;
     int nopen = 0;
-    double acc[MAXI];
+    double acc[LI];
 #pragma unroll
-    for (int k = 0; k < MAXI; ++k)
+    for (int k = 0; k < LI; ++k)
         acc[k] = (k == 0 || !(mode & CH_ACCUMULATE)) ? 0.0 : (k < p.ni ? a.totals[s + k * p.S] : 0.0);
 
     for (int c0 = 0; c0 < n; c0 += 64) {
@@ -1068,7 +1077,7 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
         const bool valid = i < n;
         int st = valid ? a.state[off + i] : 0;
         const int typ = valid ? a.type[off + i] : 0;
-        double ion[MAXI], bnd[MAXB], fre[MAXU];
+        double ion[LI], bnd[LB], fre[LU];
         if (mode & (CH_SNAPSHOT | CH_CURRENTS | CH_GATE | CH_FROM_STORED)) {
             const int node = valid ? a.node[off + i] : 0;
             const int z = p.dim == 3 ? node % p.nz : 0;
@@ -1078,10 +1087,10 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
             const double *sn = a.snap + off + (valid ? a.perm[off + i] : 0);
             const bool stored = (mode & CH_FROM_STORED) != 0;
 #pragma unroll
-            for (int j = 0; j < MAXI; ++j)
+            for (int j = 0; j < LI; ++j)
                 ion[j] = (j < p.ni) ? (stored ? sn[(size_t)j * a.nch] : (double)g[j * fs + c]) : 0.0;
 #pragma unroll
-            for (int b = 0; b < MAXB; ++b) {
+            for (int b = 0; b < LB; ++b) {
                 if (b >= p.nb) {
                     bnd[b] = 0.0;
                 } else if (p.bface[b]) {
@@ -1094,22 +1103,29 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
                 }
             }
 #pragma unroll
-            for (int u = 0; u < MAXU; ++u) fre[u] = u < p.nu ? total_at(p, u, z) : 0.0;
+            for (int u = 0; u < LU; ++u) fre[u] = u < p.nu ? total_at(p, u, z) : 0.0;
 #pragma unroll
-            for (int b = 0; b < MAXB; ++b)
-                if (b < p.nb) fre[p.bbuf[b]] -= bnd[b];
+            for (int b = 0; b < LB; ++b)
+                if (b < p.nb)
+#pragma unroll
+                    for (int u = 0; u < LU; ++u)
+                        if (u == p.bbuf[b]) fre[u] -= bnd[b];
             if ((mode & (CH_SNAPSHOT | CH_FROM_STORED)) && valid) {
-                for (int k = 0; k < p.ni; ++k) a.mouths[i + (size_t)a.M * (s + (size_t)k * p.S)] = ion[k];
-                for (int k = 0; k < p.nb; ++k)
-                    a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni) * p.S)] = bnd[k];
-                for (int k = 0; k < p.nu; ++k)
-                    a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni + p.nb) * p.S)] = fre[k];
+#pragma unroll
+                for (int k = 0; k < LI; ++k)
+                    if (k < p.ni) a.mouths[i + (size_t)a.M * (s + (size_t)k * p.S)] = ion[k];
+#pragma unroll
+                for (int k = 0; k < LB; ++k)
+                    if (k < p.nb) a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni) * p.S)] = bnd[k];
+#pragma unroll
+                for (int k = 0; k < LU; ++k)
+                    if (k < p.nu) a.mouths[i + (size_t)a.M * (s + (size_t)(k + p.ni + p.nb) * p.S)] = fre[k];
             }
         }
         if (mode & CH_CURRENTS) {
             const int sp = valid ? a.perm[off + i] : 0;
 #pragma unroll
-            for (int k = 0; k < MAXI; ++k) {
+            for (int k = 0; k < LI; ++k) {
                 double J = 0.0;
                 if (k < p.ni && valid) {
                     const double jv = q.on ? q.jnew[s + k * p.S] : a.jsr[i + (size_t)a.M * (s + (size_t)k * p.S)];
@@ -1124,8 +1140,9 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
             if (mode & CH_ACCUMULATE) {
                 const int m = (n - c0) < 64 ? (n - c0) : 64;
 #pragma unroll
-                for (int k = 0; k < MAXI; ++k)
-                    for (int q = 0; q < m; ++q) acc[k] += Jl[k][q];
+                for (int k = 0; k < LI; ++k)
+                    if (k < p.ni)
+                        for (int q = 0; q < m; ++q) acc[k] += Jl[k][q];
             }
             __syncthreads();
         }
@@ -1135,10 +1152,15 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
                 ca = (a.ca >= 0 && valid) ? a.mouths[i + (size_t)a.M * (s + (size_t)a.ca * p.S)] : 0.0;
                 mg = (a.mg >= 0 && valid) ? a.mouths[i + (size_t)a.M * (s + (size_t)a.mg * p.S)] : 0.0;
             } else {
-                ca = a.ca >= 0 ? ion[a.ca] : 0.0;
-                mg = a.mg >= 0 ? ion[a.mg] : 0.0;
+                ca = 0.0;
+                mg = 0.0;
+#pragma unroll
+                for (int j = 0; j < LI; ++j) {
+                    if (j == a.ca) ca = ion[j];
+                    if (j == a.mg) mg = ion[j];
+                }
             }
-            const double thr = valid ? channel_rate(a.model[typ], a.par + typ * NPAR, st, ca, mg) * dt : -1.0;
+            const double thr = valid ? channel_rate_m<MODEL>(a.model[typ], a.par + typ * NPAR, st, ca, mg) * dt : -1.0;
             const int end = (n - c0) < 64 ? n : c0 + 64;
             int first = c0;
             while (first < end) {
@@ -1158,7 +1180,9 @@ __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const 
             if (valid) {
                 a.state[off + i] = st;
                 const int sp = a.perm[off + i];
-                for (int k = 0; k < p.ni; ++k) a.pg[(size_t)k * a.nch + off + sp] = (double)st * a.g[typ * MAXI + k];
+#pragma unroll
+                for (int k = 0; k < LI; ++k)
+                    if (k < p.ni) a.pg[(size_t)k * a.nch + off + sp] = (double)st * a.g[typ * MAXI + k];
             }
             nopen += __popcll(__ballot(valid && st == 1));
         }
