@@ -175,8 +175,6 @@ def surrogate_leg(workloads, DyadHIP, n_sites, t_max, plain_value, heldout_sites
     kplain, kwp = spark_compare.run_plain(kspec, ks_tmax, pools=pools)
     jp = spark_compare.run_plain.jsr_low
     ks = spark_compare.compare(kplain, kwin, n_ch, ks_tmax)
-    dt = kspec.meta["dt_ms"]
-    po = [spark_compare.on_grid(a, dt) / (n_ch * ks_tmax) for a in (kplain[:, 3], kwin[:, 2])]
     jr = spark_compare.compare_jsr_lowest(kplain, kwin, jp, jq)
     out["same_tmax"] = {
         "sites": ks_sites, "t_max_ms": ks_tmax, "jsr": "depleting (SRPool)",
@@ -184,7 +182,7 @@ def surrogate_leg(workloads, DyadHIP, n_sites, t_max, plain_value, heldout_sites
         "speedup": kwp / kwq, "fallback_sites": int(kfb),
         "ks_p": {"amplitude": ks["amplitude"]["p"], "last_event": ks["last_event_ms"]["p"],
                  "open_integral": ks["open_integral"]["p"], "n_open_end": ks["n_open_end"]["p"],
-                 "open_probability": spark_compare.analysis.ks_2samp(po[0], po[1])[1],
+                 "open_probability": ks["open_probability"]["p"],
                  "jsr_lowest_other_sites": jr["others"]["p"]},
         "jsr_lowest_held_open": {"weight_p": jr["held_open_weight_p"], "drop_rel_diff": jr.get("held_open_drop_rel_diff")},
         "classes_plain": ks["classes_plain"], "classes_qs": ks["classes_qs"]}

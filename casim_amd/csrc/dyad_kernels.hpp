@@ -430,7 +430,7 @@ template <int LY> __device__ __forceinline__ double ly_total(const Params &p, in
 // away (the mirror image on the z-faces).  Writes the new grid values to G
 // (every grid field) and, at face nodes, the new face values to Fo.
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int LI_ = NI ? NI : MAXI>
-__device__ __forceinline__ void node_update(const Params &p, const T (&Cm)[NG], const T (&C0)[NG],
+__device__ __forceinline__ void node_update(const Params &p, const T *Cm, const T (&C0)[NG],
                                             const T (&Cp)[NG], const T *Lc, int sy, int zl, int zr, int z,
                                             bool face, const T (&Fv)[NF], const T (&J)[LI_], T dt, T ihx2, T ihy2, T ihz2,
                                             T (&G)[NG], T (&Fo)[NF]) {
@@ -734,8 +734,11 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 // the pool's value, which the channel call before the pass has advanced.  Chunks of one site recompute one halo plane of
 // step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
 // two steps here equal two k_sweep steps bit for bit.
+#ifndef CASIM_SWEEP2_ATTR
+#define CASIM_SWEEP2_ATTR
+#endif
 template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL, bool RING>
-__global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
+__global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
                                                 const T *__restrict__ fin, T *__restrict__ fout,
                                                 const int *__restrict__ ch_off, const int *__restrict__ snode,
                                                 const double *__restrict__ sJ, const double *__restrict__ jnew,
@@ -767,6 +770,11 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
     // where half the LDS doubles the workgroups per CU and that pays for the
     // second barrier (measured: 2D 201 x 201 sites +10 %, 3D 81 x 81 x 5 -9 %)
     constexpr bool DB = DIM == 3;
+    // with double buffers the previous plane stays in LDS for one more
+    // plane; a node's own x-1 values are read from there (only the owning
+    // thread writes them, so no barrier guards them) instead of a register
+    // ring plane per step, which frees 12 FP64 registers per plane slot
+    constexpr bool LPREV = DB;
     T *LA = lds;                                      // step-k planes
     T *LB2 = lds + (size_t)(DB ? 2 : 1) * ng * P;    // step-k+1 planes
 
@@ -806,6 +814,14 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
         const int xr = xa0 - 1 + r;
         if (xr >= 0 && xr <= p.nx - 1) load_plane(R[r], xr);
     }
+    if (LPREV && xa0 >= 1) {
+        T *Lp = LA + (size_t)((xa0 - 1) & 1) * ng * P;
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng && (fl[k] & 1)) Lp[ps.q[k] * ng + g] = R[0][k][g];
+    }
     load_face(F[0], xa0);
 
     const int chi = ch_off[s + 1];
@@ -823,13 +839,16 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
         T *Lb = LB2 + (DB ? (size_t)((x - 1) & 1) * ng * P : 0);
         // single buffer: every thread is done with the previous plane before it is overwritten
         if (!DB) lds_barrier();
+        // with LPREV every plane goes to LDS (the rim planes and plane xa0 of
+        // step k+1 are the x-1 neighbours of the next plane's update)
+        const bool wA = LPREV || interiorA, wB = LPREV ? x >= x0 : doB;
 #pragma unroll
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
                 if (g < ng && (fl[k] & 1)) {
-                    if (interiorA) La[ps.q[k] * ng + g] = R[1][k][g];
-                    if (doB) Lb[ps.q[k] * ng + g] = A[1][k][g];
+                    if (wA) La[ps.q[k] * ng + g] = R[1][k][g];
+                    if (wB) Lb[ps.q[k] * ng + g] = A[1][k][g];
                 }
         lds_barrier();
 
@@ -858,7 +877,9 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                 if (interiorA && (fl[k] & 2)) {
                     const int c = ps.q[k];
                     const int z = DIM == 3 ? c % p.nz : 0;
-                    node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], La + c * ng, p.nz * ng,
+                    const T *cm = R[0][k];
+                    if constexpr (LPREV) cm = LA + (size_t)((x - 1) & 1) * ng * P + c * ng;
+                    node_update<T, DIM, NI, NB, NG, NF, LY>(p, cm, R[1][k], R[2][k], La + c * ng, p.nz * ng,
                                                         ps.zl[k] * ng, ps.zr[k] * ng,
                                                         z, (fl[k] & 4) != 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2,
                                                         A[2][k], FA[1][k]);
@@ -935,7 +956,9 @@ __global__ __launch_bounds__(512) void k_sweep2(const T *__restrict__ gin, T *__
                 const int c = ps.q[k];
                 const int z = DIM == 3 ? c % p.nz : 0;
                 T G[NG], Fo[NF];
-                node_update<T, DIM, NI, NB, NG, NF, LY>(p, A[0][k], A[1][k], A[2][k], Lb + c * ng, p.nz * ng,
+                const T *cm = A[0][k];
+                if constexpr (LPREV) cm = LB2 + (size_t)(x & 1) * ng * P + c * ng;  // plane x-2
+                node_update<T, DIM, NI, NB, NG, NF, LY>(p, cm, A[1][k], A[2][k], Lb + c * ng, p.nz * ng,
                                                     ps.zl[k] * ng, ps.zr[k] * ng, z,
                                                     (fl[k] & 4) != 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
                 const size_t o = (size_t)xb * P + c;

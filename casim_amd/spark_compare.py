@@ -122,11 +122,14 @@ def on_grid(t, dt):
 
 
 def compare(plain, qs_win, n_channels, t_max, alpha=0.01, dt=workloads.DT_MS):
-    """KS tests of amplitude, last event time, integral of N dt and N at t_max."""
+    """KS tests of amplitude, last event time, integral of N dt, N at t_max
+    and open probability (integral / (n_RyR t_max))."""
+    n_ch = np.asarray(n_channels, dtype=np.float64)
     cols = {"amplitude": (plain[:, 0], qs_win[:, 0]),
             "last_event_ms": (on_grid(plain[:, 2], dt), on_grid(qs_win[:, 1], dt)),
             "open_integral": (on_grid(plain[:, 3], dt), on_grid(qs_win[:, 2], dt)),
-            "n_open_end": (plain[:, 5], qs_win[:, 3])}
+            "n_open_end": (plain[:, 5], qs_win[:, 3]),
+            "open_probability": (on_grid(plain[:, 3], dt) / (n_ch * t_max), on_grid(qs_win[:, 2], dt) / (n_ch * t_max))}
     out = {}
     for name, (a, b) in cols.items():
         d, p = analysis.ks_2samp(a, b)
