@@ -366,6 +366,20 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Element at a 32-bit byte offset from a wave-uniform base: the address is
+// base + zext(offset), which the compiler issues as a global access with the
+// base in scalar registers and one 32-bit vector offset, instead of a 64-bit
+// address per lane and field.  Offsets stay below 4 GiB because they never
+// leave one site's field (NS nodes).
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, unsigned boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T *base, unsigned boff, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
 // Species layouts the sweeps can take at compile time: bound field b binds
 // ion bion[b], belongs to buffer bbuf[b], lives on the z-faces only when
 // bface[b] and sits in grid (or face) slot bslot[b]; ufaces[u]: buffer u
@@ -630,7 +644,7 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && ps.act[k]) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
+                if (g < ng && ps.act[k]) D[k][g] = ld_off(gs + g * fs, (unsigned)((x * P + ps.q[k]) * (int)sizeof(T)));
     };
     auto load_face = [&](T (&D)[SL][NF], int x) {
 #pragma unroll
@@ -638,8 +652,8 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 #pragma unroll
             for (int f = 0; f < NF; ++f)
                 if (f < nfc && ps.fz[k] >= 0)
-                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
-                                  ps.yy[k]];
+                    D[k][f] = ld_off(fin + ((size_t)f * p.S + s) * 2 * p.nFP,
+                                     (unsigned)((ps.fz[k] * p.nFP + x * p.ny + ps.yy[k]) * (int)sizeof(T)));
     };
 #pragma unroll
     for (int r = 0; r < RS - 1; ++r)
@@ -691,16 +705,16 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
             node_update<T, DIM, NI, NB, NG, NF, LY>(p, R[0][k], R[1][k], R[2][k], L + c * ng, p.nz * ng,
                                                 ps.zl[k] * ng, ps.zr[k] * ng, z,
                                                 ps.fz[k] >= 0, F[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
-            const size_t o = (size_t)x * P + c;
+            const unsigned o = (unsigned)((x * P + c) * (int)sizeof(T));
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng) os[g * fs + o] = G[g];
+                if (g < ng) st_off(os + g * fs, o, G[g]);
             if (ps.fz[k] >= 0)
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
                     if (f < nfc)
-                        fout[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
-                             ps.yy[k]] = Fo[f];
+                        st_off(fout + ((size_t)f * p.S + s) * 2 * p.nFP,
+                               (unsigned)((ps.fz[k] * p.nFP + x * p.ny + ps.yy[k]) * (int)sizeof(T)), Fo[f]);
         }
         // advance the rings by one plane
 #pragma unroll
@@ -797,7 +811,8 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
         for (int k = 0; k < SL; ++k)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g < ng && (fl[k] & 1)) D[k][g] = gs[g * fs + (size_t)x * P + ps.q[k]];
+                if (g < ng && (fl[k] & 1))
+                    D[k][g] = ld_off(gs + g * fs, (unsigned)((x * P + ps.q[k]) * (int)sizeof(T)));
     };
     auto load_face = [&](T (&D)[SL][NF], int x) {
 #pragma unroll
@@ -805,8 +820,8 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
 #pragma unroll
             for (int f = 0; f < NF; ++f)
                 if (f < nfc && ps.fz[k] >= 0)
-                    D[k][f] = fin[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)x * p.ny +
-                                  ps.yy[k]];
+                    D[k][f] = ld_off(fin + ((size_t)f * p.S + s) * 2 * p.nFP,
+                                     (unsigned)((ps.fz[k] * p.nFP + x * p.ny + ps.yy[k]) * (int)sizeof(T)));
     };
     // planes xa0-1 .. xa0+1 (plane -1 does not exist: A copies the rim plane 0)
 #pragma unroll
@@ -828,6 +843,11 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
     int curA = first_channel_at(snode, ch_off[s], chi, xa0 * P);
     int curB = first_channel_at(snode, ch_off[s], chi, x0 * P);
 
+    // the prologue's loads land before the loop: registers entering it then
+    // carry no pending load, and the compiler's wait counts inside the loop
+    // cover only the plane prefetch (otherwise it waits for every load in
+    // flight, the prefetch included, before the first use of a carried value)
+    __builtin_amdgcn_s_waitcnt(0);
     for (int x = xa0; x <= x1; ++x) {
 #pragma unroll
         for (int k = 0; k < SL; ++k) asm volatile("" : "+v"(fl[k]));
@@ -961,16 +981,16 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
                 node_update<T, DIM, NI, NB, NG, NF, LY>(p, cm, A[1][k], A[2][k], Lb + c * ng, p.nz * ng,
                                                     ps.zl[k] * ng, ps.zr[k] * ng, z,
                                                     (fl[k] & 4) != 0, FA[0][k], J[k], dt, ihx2, ihy2, ihz2, G, Fo);
-                const size_t o = (size_t)xb * P + c;
+                const unsigned o = (unsigned)((xb * P + c) * (int)sizeof(T));
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
-                    if (g < ng) os[g * fs + o] = G[g];
+                    if (g < ng) st_off(os + g * fs, o, G[g]);
                 if (fl[k] & 4)
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
                         if (f < nfc)
-                            fout[((size_t)f * p.S + s) * 2 * p.nFP + (size_t)ps.fz[k] * p.nFP + (size_t)xb * p.ny +
-                                 ps.yy[k]] = Fo[f];
+                            st_off(fout + ((size_t)f * p.S + s) * 2 * p.nFP,
+                                   (unsigned)((ps.fz[k] * p.nFP + xb * p.ny + ps.yy[k]) * (int)sizeof(T)), Fo[f]);
             }
         }
         // advance the rings by one plane
