@@ -17,6 +17,7 @@ constexpr int QS_NEAR = 16;
 constexpr int QS_F = 8 + 3 * QS_NEAR;  // geometric features per RyR
 constexpr int QS_FMAX = QS_F + 1;      // + the linear-superposition feature
 constexpr int QS_MAXCH = 128;
+constexpr int QS_MAXMODES = 8;  // stationary model: eigenmodes (buffers + 1)
 
 // Per-site statistics: the event record's columns (QS_AMP .. QS_EVENTS) and
 // the same quantities cut at t_max (QS_W*), for comparison with fixed-step runs.
@@ -57,6 +58,11 @@ struct QsDev {
     float *ca;       // [channels] mouth [Ca2+] in use
     float *spread;   // [channels]
     float *ovr;      // [channels] supplied by the full solver
+    // the reference's stationary model (LinearNaraghiNeher + StationaryDiffusionModel)
+    // in place of the ensemble when st_n > 0
+    const double *xd, *yd;  // [channels] positions in the coordinate file's units
+    int st_n;
+    double st_coef[QS_MAXMODES], st_decay[QS_MAXMODES], st_bg, st_open;
     int model;
     double par[NPAR];
     double mg;
@@ -74,6 +80,30 @@ __device__ __forceinline__ float lin_added(const QsDev &q, float r2) {
     const int k = (int)r2;
     const float w = r2 - (float)k;
     return q.lin_tab[k] + w * (q.lin_tab[k + 1] - q.lin_tab[k]);
+}
+
+// StationaryDiffusionModel::GetConcentrations (StationaryDiffusionModel.cpp:3-11)
+// for channel i of a site: over the open channels j in order, the [Ca2+]
+// LinearNaraghiNeher's table holds for their distance (LinearNaraghiNeher.cpp:
+// 62-71: the modes' sum over r, "Open Concentration" at r <= 1e-6), then the
+// background.  FP64 throughout, as the reference.
+__device__ double stationary_mouth(const QsDev &q, int off, int n, int i) {
+#pragma clang fp contract(off)
+    const double xi = q.xd[off + i], yi = q.yd[off + i];
+    double c = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (q.state[off + j] != 1) continue;  // isOpen() * added adds +0
+        const double dx = q.xd[off + j] - xi, dy = q.yd[off + j] - yi;
+        const double r = sqrt(dx * dx + dy * dy + 0.0);  // Point::d, z = 0
+        double a = 0.0;
+        if (r > 1e-6) {
+            for (int m = 0; m < q.st_n; ++m) a += 1.0 / r * q.st_coef[m] * exp(-r * q.st_decay[m]);
+        } else {
+            a = q.st_open;
+        }
+        c += a;
+    }
+    return c + q.st_bg;
 }
 
 // Features of every channel of every active site: lanes over channels, the
@@ -184,6 +214,16 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     for (int i = lane; i < n; i += 64) {
         const size_t row = off + i;
         float ca, sp;
+        if (q.st_n > 0) {
+            // the reference's stationary model: rates from the FP64 value
+            const double cd = stationary_mouth(q, off, n, i);
+            q.ca[row] = (float)cd;
+            q.spread[row] = 0.f;
+            spv[i] = 0.f;
+            cav[i] = (float)cd;
+            rate[i] = channel_rate(q.model, q.par, q.state[row], cd, q.mg);
+            continue;
+        }
         if (wait == 2) {
             ca = q.ovr[row];
             sp = 0.f;
@@ -497,6 +537,14 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.lin_residual = 0;
         q.ca = h->alloc<float>(h->nch);
         q.spread = h->alloc<float>(h->nch);
+        {
+            double *xd = h->alloc<double>(h->nch), *yd = h->alloc<double>(h->nch);
+            HIPCK(hipMemcpyAsync(xd, c->channel_x, sizeof(double) * h->nch, hipMemcpyHostToDevice, h->stream));
+            HIPCK(hipMemcpyAsync(yd, c->channel_y, sizeof(double) * h->nch, hipMemcpyHostToDevice, h->stream));
+            q.xd = xd;
+            q.yd = yd;
+            q.st_n = 0;
+        }
         q.ovr = h->alloc<float>(h->nch);
         q.out = nullptr;
         q.model = c->model;
@@ -567,20 +615,24 @@ int casim_qs_reset(CasimQS *h) {
 int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch, int32_t *rounds,
                  int32_t *n_active, int32_t *n_waiting) {
     return guard([&] {
-        REQUIRE(h && m, CASIM_E_INVALID, "null handle");
-        REQUIRE(mlp_inputs(m) == h->q.nf, CASIM_E_INVALID,
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        const bool stationary = h->q.st_n > 0;
+        REQUIRE(m || stationary, CASIM_E_INVALID, "null ensemble (and no stationary model set)");
+        REQUIRE(stationary || mlp_inputs(m) == h->q.nf, CASIM_E_INVALID,
                 h->q.lin_n ? "with a superposition table the surrogate must take 57 features"
                            : "the surrogate must take 56 features (57 with a superposition table)");
         h->use();
-        const int E = mlp_members(m);
-        if (!h->q.out || h->q.members != E) {
-            if (h->q.out) (void)hipFree(h->q.out);
-            h->q.out = nullptr;
-            HIPCK(hipMalloc(&h->q.out, sizeof(float) * (size_t)E * h->nch));
+        if (!stationary) {
+            const int E = mlp_members(m);
+            if (!h->q.out || h->q.members != E) {
+                if (h->q.out) (void)hipFree(h->q.out);
+                h->q.out = nullptr;
+                HIPCK(hipMalloc(&h->q.out, sizeof(float) * (size_t)E * h->nch));
+            }
+            h->q.members = E;
+            h->q.ym = mlp_out_mean(m);
+            h->q.ys = mlp_out_std(m);
         }
-        h->q.members = E;
-        h->q.ym = mlp_out_mean(m);
-        h->q.ys = mlp_out_std(m);
         const int batch = std::max(1, (int)wait_batch);
         int cnt[4] = {0, 0, 0, 0};
         auto plan = [&] {
@@ -593,9 +645,11 @@ int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch
         int r = 0;
         while (r < max_rounds && cnt[0] > 0 && cnt[1] < batch && cnt[1] < cnt[0]) {
             h->q.R = cnt[2];
-            hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
-            HIPCK(hipGetLastError());
-            mlp_forward_device(m, h->q.X, cnt[2], h->q.out, h->stream);
+            if (!stationary) {
+                hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+                HIPCK(hipGetLastError());
+                mlp_forward_device(m, h->q.X, cnt[2], h->q.out, h->stream);
+            }
             hipLaunchKernelGGL(k_qs_event, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
             HIPCK(hipGetLastError());
             ++r;
@@ -688,6 +742,31 @@ int casim_qs_set_linear_feature(CasimQS *h, const float *added, int32_t n, float
             h->q.lin_residual = residual != 0;
             h->q.nf = QS_F + 1;
         }
+    });
+}
+
+int casim_qs_set_stationary_model(CasimQS *h, int32_t n_modes, const double *coef, const double *decay,
+                                  double background, double ca_open) {
+    return guard([&] {
+        REQUIRE(h, CASIM_E_INVALID, "null handle");
+        REQUIRE(n_modes >= 0 && n_modes <= QS_MAXMODES, CASIM_E_RANGE,
+                "a stationary model takes 0..8 modes (up to 7 buffers)");
+        REQUIRE(n_modes == 0 || (coef && decay), CASIM_E_INVALID, "null mode arrays");
+        REQUIRE(n_modes == 0 || !h->q.deplete, CASIM_E_INVALID,
+                "the stationary model does not follow the jSR: create the sites without jsr_deplete");
+        for (int k = 0; k < n_modes; ++k)
+            REQUIRE(std::isfinite(coef[k]) && std::isfinite(decay[k]) && decay[k] >= 0, CASIM_E_INVALID,
+                    "mode coefficients must be finite and decays >= 0");
+        REQUIRE(std::isfinite(background) && std::isfinite(ca_open), CASIM_E_INVALID, "non-finite concentration");
+        h->use();
+        HIPCK(hipStreamSynchronize(h->stream));
+        h->q.st_n = n_modes;
+        for (int k = 0; k < QS_MAXMODES; ++k) {
+            h->q.st_coef[k] = k < n_modes ? coef[k] : 0.0;
+            h->q.st_decay[k] = k < n_modes ? decay[k] : 0.0;
+        }
+        h->q.st_bg = background;
+        h->q.st_open = ca_open;
     });
 }
 

@@ -32,6 +32,8 @@ def _declare(lib):
         "casim_qs_get_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "casim_qs_get_features": (C.c_int, [P, _fp]),
         "casim_qs_set_linear_feature": (C.c_int, [P, _fp, C.c_int32, C.c_float, C.c_float, C.c_int32]),
+        "casim_qs_set_stationary_model": (C.c_int, [P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                    C.c_double, C.c_double]),
         "casim_qs_get_window_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "casim_qs_get_jsr": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "casim_qs_record_open_counts": (C.c_int, [P, C.c_double, C.c_int32]),

@@ -249,10 +249,14 @@ class QuasiStationarySites:
     solver's mouth values instead (RunSimulation's `fallback`)."""
 
     def __init__(self, spec, jsr_site, t_max, net, spread_threshold=0.02, mg=0.0, rng_states=None, pools=None,
-                 lin=None):
+                 lin=None, stationary=None):
         """pools: a config.PoolSpec with jsr_constant False makes the jSR
         deplete between events (SRPool's pool, calcium only); jSR starts at
-        jsr_site."""
+        jsr_site.  stationary: a stationary.StationaryModel (the reference's
+        LinearNaraghiNeher) that gives the mouth values instead of the
+        ensemble (net may then be None); it does not follow the jSR."""
+        if net is None and stationary is None:
+            raise ValueError("QuasiStationarySites needs a surrogate ensemble or a stationary model")
         self.lib = mlp.lib()
         self.spec, self.net = spec, net
         self.S = spec.n_sites
@@ -288,6 +292,13 @@ class QuasiStationarySites:
             tab = np.ascontiguousarray(lin.added, dtype=np.float32)
             abi.check(self.lib.casim_qs_set_linear_feature(self.h, mlp.fptr(tab), len(tab), float(lin.background),
                                                            float(lin.jsr_ref), int(lin.residual)))
+        self.stationary = stationary
+        if stationary is not None:
+            coef = np.ascontiguousarray(stationary.coef, dtype=np.float64)
+            decay = np.ascontiguousarray(stationary.decay, dtype=np.float64)
+            self._keep_st = (coef, decay)
+            abi.check(self.lib.casim_qs_set_stationary_model(self.h, len(coef), abi.dptr(coef), abi.dptr(decay),
+                                                             float(stationary.background), float(stationary.ca_open)))
         self.fallback_solves = 0
         self.fallback_s = 0.0
         self.rounds = 0
@@ -310,7 +321,8 @@ class QuasiStationarySites:
         """Up to max_rounds events per active, non-waiting site;
         (rounds run, active sites, waiting sites)."""
         r, a, w = C.c_int32(), C.c_int32(), C.c_int32()
-        abi.check(self.lib.casim_qs_run(self.h, self.net.h, int(max_rounds), int(wait_batch), C.byref(r),
+        net = self.net.h if self.stationary is None else None
+        abi.check(self.lib.casim_qs_run(self.h, net, int(max_rounds), int(wait_batch), C.byref(r),
                                         C.byref(a), C.byref(w)))
         return r.value, a.value, w.value
 

@@ -146,7 +146,9 @@ int casim_qs_reset(CasimQS *q);
  * for the full solver.  Returns early when no site is active, when
  * wait_batch sites wait, or when every active site waits.  rounds, n_active
  * and n_waiting receive the rounds run and the counts after the last round
- * (any may be NULL).  Calls on one handle must not overlap. */
+ * (any may be NULL).  Calls on one handle must not overlap.  With a
+ * stationary model set (casim_qs_set_stationary_model) m may be NULL: the
+ * mouth values come from that model and no site waits. */
 int casim_qs_run(CasimQS *q, CasimMLP *m, int32_t max_rounds, int32_t wait_batch, int32_t *rounds,
                  int32_t *n_active, int32_t *n_waiting);
 /* Sites waiting for the full solver: flags [S] (1 = waiting) and the
@@ -172,6 +174,21 @@ int casim_qs_get_window_stats(CasimQS *q, double *out);
  * mouth value is exp(prediction + feature). */
 int casim_qs_set_linear_feature(CasimQS *q, const float *added, int32_t n, float background, float jsr_ref,
                                 int32_t residual);
+
+/* The reference's own mouth model in place of the ensemble: the
+ * StationaryDiffusionModel that QuasyStationaryReleaseSite builds
+ * (QuasyStationaryReleaseSite.cpp:48-49, StationaryDiffusionModel.cpp:3-11)
+ * with LinearNaraghiNeher's table (LinearNaraghiNeher.cpp:9-75).  The mouth
+ * [Ca2+] of channel j is background + sum over open channels i, in index
+ * order, of added(r_ij), r_ij the distance of the channel_x / channel_y
+ * positions given at creation, added(r) = sum_m coef[m] exp(-r decay[m]) / r
+ * for r > 1e-6 and ca_open otherwise; FP64.  coef and decay are the table's
+ * eigenmodes with the unit factors folded in (coef = aa * eigv[n][m] * 1e-4,
+ * decay = sqrt|mu_m| * 1e-8; the host computes them, casim_amd/stationary.py).
+ * n_modes <= 8; 0 turns the model off.  Not with a depleting jSR (the model
+ * does not follow the jSR, as in the reference): CASIM_E_INVALID. */
+int casim_qs_set_stationary_model(CasimQS *q, int32_t n_modes, const double *coef, const double *decay,
+                                  double background, double ca_open);
 
 /* Time-histogram input of IndChannelCREAnalyzer::RunAnalysis
  * (IndChannelCREAnalyzer.cpp:69-117): at each event at time t, bins from the

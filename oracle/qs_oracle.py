@@ -136,6 +136,84 @@ def run_site(n, init_channel, rng_state, model, params, mg, t_max, mouth_fn, poo
     return rec, stream.state
 
 
+FAR = 96485.33e-6  # StationaryDiffusionModel.h: Far
+PI = math.acos(-1.0)
+
+
+def linear_naraghi_neher(buffers, current, ca0, dca):
+    """LinearNaraghiNeher.cpp:9-64: the linearised steady state of one point
+    source with mobile buffers.  buffers: {name: {"kon", "koff", "D",
+    "Total Concentration"}} (iterated in name order, like the std::map).
+    Per buffer k = B K / (K + ca0)^2, tau = 1 / (koff + kon ca0); with
+    Dm = diag(D_1 .. D_n, dca) and AA the (n+1) x (n+1) rate matrix of
+    (bound_1 .. bound_n, Ca), B = -Dm^-1 AA; its eigenpairs (mu, V) (real
+    parts; the smallest eigenvalue moved last, as the reference does) and
+    aa = V^-1 u, u = (0 .. 0, current / (4 pi Far dca)).  Returns (aa, mu, V)."""
+    k, tau, d = [], [], []
+    for name in sorted(buffers):
+        b = buffers[name]
+        K = b["koff"] / b["kon"]
+        k.append(b["Total Concentration"] * K / ((K + ca0) * (K + ca0)))
+        tau.append(1.0 / (b["koff"] + b["kon"] * ca0))
+        d.append(b["D"])
+    n = len(k)
+    Dm = np.zeros((n + 1, n + 1))
+    for i in range(n):
+        Dm[i, i] = d[i]
+    Dm[n, n] = dca
+    AA = np.zeros((n + 1, n + 1))
+    for i in range(n):
+        AA[i, i] = -1.0 / tau[i]
+        AA[i, n] = k[i] / tau[i]
+        AA[n, i] = -AA[i, i]
+        AA[n, n] -= AA[i, n]
+    B = -np.linalg.inv(Dm) @ AA
+    w, v = np.linalg.eig(B)
+    mu, V = np.real(w).copy(), np.real(v).copy()
+    j = int(np.argmin(mu))
+    mu[[j, n]] = mu[[n, j]]
+    V[:, [j, n]] = V[:, [n, j]]
+    u = np.zeros(n + 1)
+    u[n] = current / (4 * PI * FAR * dca)
+    aa = np.linalg.solve(V, u)
+    return aa, mu, V
+
+
+def nn_added(aa, mu, V, r, ca_open):
+    """One entry of added_concentrations (LinearNaraghiNeher.cpp:66-71): the
+    [Ca2+] an open channel adds at distance r, ca_open at its own mouth."""
+    if not r > 1e-6:
+        return ca_open
+    n = len(aa) - 1
+    dca = 0.0
+    for ii in range(n + 1):
+        dca += 1 / r * aa[ii] * math.exp(-r * math.sqrt(abs(mu[ii])) * 1e-8) * V[n, ii] * 1e-4
+    return dca
+
+
+def stationary_table(xy, aa, mu, V, ca_open):
+    """added_concentrations[i][j] over the channel positions xy [n, 2]
+    (Point::d with z = 0)."""
+    n = len(xy)
+    tab = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            dx, dy = xy[i][0] - xy[j][0], xy[i][1] - xy[j][1]
+            tab[i, j] = nn_added(aa, mu, V, math.sqrt(dx * dx + dy * dy + 0.0), ca_open)
+    return tab
+
+
+def stationary_concentrations(tab, states, background):
+    """StationaryDiffusionModel::GetConcentrations (StationaryDiffusionModel.cpp:3-11)."""
+    n = len(states)
+    out = [0.0] * n
+    for j in range(n):
+        for i in range(n):
+            out[j] += float(states[i] == 1) * tab[i, j]
+        out[j] += background
+    return out
+
+
 def analyze(rec):
     """amplitude, TTP, TTE, Q, Q/TTE and the number of events of one record
     (IndChannelCREAnalyzer.cpp:69-117: amp / ttp from the first strict

@@ -422,3 +422,73 @@ def test_trained_surrogate_beats_linear_superposition(gpu):
     assert got["rel_mean"] < 0.025
     assert got["rel_mean"] < 0.25 * base["rel_mean"]
     assert got["rel_mean_above_5uM"] < 0.5 * base["rel_mean_above_5uM"]
+
+
+def _stationary_from_oracle(buffers, current, ca0, dca, ca_open):
+    """The device model built from the oracle's modes, so that the GPU test
+    checks the device arithmetic alone (the host's own mode computation is
+    checked against the oracle in tests/test_stationary.py)."""
+    from casim_amd import stationary
+    aa, mu, V = Q.linear_naraghi_neher(buffers, current, ca0, dca)
+    n = len(aa) - 1
+    return (aa, mu, V), stationary.StationaryModel(coef=aa * V[n, :] * 1e-4, decay=np.sqrt(np.abs(mu)) * 1e-8,
+                                                   background=ca0, ca_open=ca_open)
+
+
+ST_BUFFERS = {"ATP": {"kon": 1e3, "koff": 1e5, "D": 1e-7, "Total Concentration": 50.0},
+              "Calmodulin": {"kon": 5e2, "koff": 2e3, "D": 2e-7, "Total Concentration": 24.0}}
+
+
+@pytest.mark.gpu
+def test_stationary_model_whole_run_matches_oracle(gpu):
+    """QuasyStationaryReleaseSite with the reference's LinearNaraghiNeher mouth
+    model (QuasyStationaryReleaseSite.cpp:48-49, 89-133): every site's channel
+    history, release statistics and time bins against the oracle's event loop
+    with StationaryDiffusionModel::GetConcentrations at each event."""
+    from casim_amd import surrogate
+    spec = _spec(24)
+    modes, model = _stationary_from_oracle(ST_BUFFERS, 10.0, 0.1, 2.2e-6, 150.0)
+    qs = surrogate.QuasiStationarySites(spec, np.full(spec.n_sites, 1000.0), 2.0, None, stationary=model)
+    qs.RecordOpenCounts(0.01, 201)
+    stats = qs.RunSimulation(fallback=False)
+    win = qs.GetWindowStats()
+    bins = qs.GetOpenCounts()
+    states = qs.GetChannelsStates()
+    mouths = qs.GetMouths()[0]
+    par = _par(spec)
+    for s, site in enumerate(spec.sites):
+        n = len(site)
+        xy = np.array([[c[0], c[1]] for c in site])
+        tab = Q.stationary_table(xy, *modes, 150.0)
+        seen = []
+
+        def mouth(st, jsr):
+            c = Q.stationary_concentrations(tab, st, 0.1)
+            seen.append(c)
+            return c
+
+        rec, _ = Q.run_site(n, spec.init_channel[s], spec.rng_states[s], SHEEP, par, 0.0, 2.0, mouth)
+        want = Q.analyze(rec)
+        assert np.array_equal(states[s, :n], rec[-1][1]), s
+        np.testing.assert_array_equal(stats[s, [0, 5]], want[[0, 5]])
+        np.testing.assert_allclose(stats[s, 1:5], want[1:5], rtol=1e-12, atol=1e-15)
+        assert np.array_equal(bins[:, s], Q.open_count_bins(rec, 0.01, 201)), s
+        ww = Q.analyze_window(rec, 2.0)
+        np.testing.assert_array_equal(win[s, [0, 3]], ww[[0, 3]])
+        np.testing.assert_allclose(win[s, 1:3], ww[1:3], rtol=1e-12, atol=1e-15)
+        # the mouth values of the last event the device evaluated
+        np.testing.assert_allclose(mouths[_site_rows(qs, s)], np.float32(seen[-1]), rtol=1e-6)
+    assert stats[:, 5].sum() > 3 * spec.n_sites  # the runs are not trivial
+    assert stats[:, 0].max() >= 3  # open sets of several channels occur
+    qs.close()
+
+
+@pytest.mark.gpu
+def test_stationary_model_is_refused_with_a_depleting_jsr(gpu):
+    from casim_amd import abi as A
+    from casim_amd import spark_compare, surrogate
+    spec = _spec(4)
+    _, model = _stationary_from_oracle(ST_BUFFERS, 10.0, 0.1, 2.2e-6, 150.0)
+    with pytest.raises(A.CasimError, match="jsr_deplete"):
+        surrogate.QuasiStationarySites(spec, np.full(spec.n_sites, 1000.0), 2.0, None, stationary=model,
+                                       pools=spark_compare.depleting_pool())
