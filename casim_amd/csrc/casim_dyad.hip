@@ -242,6 +242,9 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         *out = nullptr;
         REQUIRE(c->dim == 2 || c->dim == 3, CASIM_E_INVALID, "dim must be 2 or 3");
         REQUIRE(c->nx >= 5 && c->ny >= 5, CASIM_E_INVALID, "grid needs nx, ny >= 5");
+        // the sweeps address a site's field with a 32-bit byte offset
+        REQUIRE((double)c->nx * c->ny * (c->dim == 3 ? c->nz : 1) * 8.0 < 2147483648.0, CASIM_E_INVALID,
+                "a site's grid must stay below 2^28 nodes");
         REQUIRE(c->dim == 2 || c->nz >= 2, CASIM_E_INVALID, "3D grid needs nz >= 2");
         REQUIRE(c->n_sites >= 1 && c->n_sites <= 65535, CASIM_E_INVALID, "n_sites must be in 1..65535");
         REQUIRE(c->n_ions >= 1 && c->n_ions <= MAXI, CASIM_E_INVALID, "n_ions out of range (1..4)");

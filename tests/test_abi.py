@@ -42,6 +42,10 @@ def test_invalid_configs_are_rejected():
     cfg, keep = sp.to_struct()
     cfg.n_ions = 9
     assert lib.casim_dyad_create(C.byref(cfg), C.byref(h)) == -1
+    cfg, keep = sp.to_struct()
+    cfg.nx, cfg.ny, cfg.nz = 4096, 4096, 20  # 2^28.3 nodes: beyond the sweeps' 32-bit offsets
+    assert lib.casim_dyad_create(C.byref(cfg), C.byref(h)) == -1
+    assert b"2^28" in lib.casim_last_error()
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a device is present")
