@@ -426,3 +426,21 @@ def test_pools_run_one_step_per_pass(gpu):
     assert dev.StepsPerPass() == 2
     dev.AttachPools(config.pools_from_json(j, sp))
     assert dev.StepsPerPass() == 1
+
+
+def test_channel_count_edges(gpu, oracle_mod):
+    """Sites with 1, 63, 64, 65 and 127 RyRs (the channel kernel works in
+    chunks of 64 lanes; 127 is the per-site maximum), stochastic FP64."""
+    sp = workloads.batched_dyads(n_sites=5, geometry_seed=3, extent_nm=440.0)
+    c = 220.0
+    cells = [(c + 31.0 * (i - 5.5), c + 31.0 * (j - 5.5), 0) for i in range(12) for j in range(12)]
+    # nearest the centre first, so every site keeps a compact cluster
+    cells.sort(key=lambda p: ((p[0] - c) ** 2 + (p[1] - c) ** 2, p[0], p[1]))
+    sp.sites = [cells[:n] for n in (1, 63, 64, 65, 127)]
+    ora, dev, _ = setup_pair(oracle_mod, sp)
+    dt = 1e-5
+    dev.Step(dt, 300)
+    ora.step(dt, 300)
+    assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
+    assert np.array_equal(dev.GetRngStates(), ora.rng_states())
+    compare(ora, dev, sp, FP64_TOL)
