@@ -31,10 +31,11 @@ def log(*a):
 
 
 def run_gpu_leg(workloads, DyadHIP, precision, n_sites, steps, warmup, chunk, retune=False, tuning=None,
-                steps_per_pass=2):
+                steps_per_pass=2, split=True):
     from casim_amd import autotune
     sp = workloads.batched_dyads(n_sites=n_sites, precision=precision)
     dev = DyadHIP(sp)
+    dev.SetSplit(split)
     dev.SetStepsPerPass(steps_per_pass)
     dev.InitOpeningAll()
     ri, rb = sp.rim_arrays()
@@ -276,6 +277,9 @@ def main():
     ap.add_argument("--tuning", default=None, help="planes,threads of the state sweep (skips the autotuner)")
     ap.add_argument("--steps-per-pass", type=int, default=2, choices=[1, 2],
                     help="2: pairs of steps per state pass where possible (default); 1: one pass per step")
+    ap.add_argument("--no-split", action="store_true",
+                    help="one stream for the whole batch (default: two half-batch streams, channel calls of one "
+                         "half beside the other half's pass)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--qs-tmax", type=float, default=20.0, help="simulated ms of the surrogate leg")
     ap.add_argument("--no-coupled", action="store_true")
@@ -298,13 +302,13 @@ def main():
         return
     log("fp64 leg: %d sites, %d steps" % (a.sites, a.steps))
     r64 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP64, a.sites, a.steps, a.warmup, a.chunk, a.retune, tuning,
-                      a.steps_per_pass)
+                      a.steps_per_pass, not a.no_split)
     log("fp64:", json.dumps(r64))
     r32 = None
     if not a.no_fp32:
         log("fp32 leg")
         r32 = run_gpu_leg(workloads, DyadHIP, abi.CASIM_FP32, a.sites, a.steps, a.warmup, a.chunk, a.retune,
-                          tuning, a.steps_per_pass)
+                          tuning, a.steps_per_pass, not a.no_split)
         log("fp32:", json.dumps(r32))
     sur = None
     if not a.no_surrogate:
@@ -343,7 +347,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": r64["workload"], "sites": a.sites, "grid": r64["grid"], "dt_ns": 10,
-                   "path": "plain batched RD, graph of %d steps" % min(a.chunk, a.steps),
+                   "path": "plain batched RD, graph of %d steps%s" % (
+                       min(a.chunk, a.steps), "" if a.no_split else ", two half-batch streams"),
                    "tuning": r64["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -126,6 +126,7 @@ def _declare(lib):
         "casim_dyad_bytes_per_site_step": (C.c_double, [P]),
         "casim_dyad_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32]),
         "casim_dyad_set_steps_per_pass": (C.c_int, [P, C.c_int32]),
+        "casim_dyad_set_split": (C.c_int, [P, C.c_int32]),
         "casim_dyad_record_open_counts": (C.c_int, [P, C.c_int32, C.c_int32]),
         "casim_dyad_get_open_counts": (C.c_int, [P, C.c_void_p]),
         "casim_dyad_steps_per_pass": (C.c_int32, [P]),

@@ -144,20 +144,23 @@ void CasimDyad::launch_pair(double dt) {
 }
 
 void CasimDyad::launch_channels(int mode, double dt) {
+    const dim3 grid(range_count());
+    hipStream_t st = range_st();
+    const int s0 = range_s0;
     if (precision == CASIM_FP64) {
         if (chan_small)
-            hipLaunchKernelGGL((k_channels<double, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), dim3(P.S), dim3(64), 0, stream,
-                               (const double *)g[cur], (const double *)f[cur], Ch, P, Q, mode, dt);
+            hipLaunchKernelGGL((k_channels<double, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), grid, dim3(64), 0, st,
+                               (const double *)g[cur], (const double *)f[cur], Ch, P, Q, mode, dt, s0);
         else
-            hipLaunchKernelGGL((k_channels<double, 0, 0, 0, 0>), dim3(P.S), dim3(64), 0, stream, (const double *)g[cur],
-                               (const double *)f[cur], Ch, P, Q, mode, dt);
+            hipLaunchKernelGGL((k_channels<double, 0, 0, 0, 0>), grid, dim3(64), 0, st, (const double *)g[cur],
+                               (const double *)f[cur], Ch, P, Q, mode, dt, s0);
     } else {
         if (chan_small)
-            hipLaunchKernelGGL((k_channels<float, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), dim3(P.S), dim3(64), 0, stream,
-                               (const float *)g[cur], (const float *)f[cur], Ch, P, Q, mode, dt);
+            hipLaunchKernelGGL((k_channels<float, CASIM_MODEL_SHEEP_CANNELL, 1, 3, 3>), grid, dim3(64), 0, st,
+                               (const float *)g[cur], (const float *)f[cur], Ch, P, Q, mode, dt, s0);
         else
-            hipLaunchKernelGGL((k_channels<float, 0, 0, 0, 0>), dim3(P.S), dim3(64), 0, stream, (const float *)g[cur],
-                               (const float *)f[cur], Ch, P, Q, mode, dt);
+            hipLaunchKernelGGL((k_channels<float, 0, 0, 0, 0>), grid, dim3(64), 0, st, (const float *)g[cur],
+                               (const float *)f[cur], Ch, P, Q, mode, dt, s0);
     }
     HIPCK(hipGetLastError());
 }
@@ -433,6 +436,10 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         }
 
         HIPCK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+        HIPCK(hipStreamCreateWithFlags(&d->stream2, hipStreamNonBlocking));
+        HIPCK(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
+        HIPCK(hipEventCreateWithFlags(&d->ev_off, hipEventDisableTiming));
+        HIPCK(hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) {
             HIPCK(hipMalloc(&d->g[k], std::max<size_t>(1, d->grid_elems()) * d->tsize));
             HIPCK(hipMalloc(&d->f[k], std::max<size_t>(1, d->face_elems()) * d->tsize));
@@ -511,6 +518,7 @@ int casim_dyad_destroy(CasimDyad *d) {
     if (!d) return CASIM_OK;
     (void)hipSetDevice(d->dev);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
+    if (d->stream2) (void)hipStreamSynchronize(d->stream2);
     d->clear_graphs();
     casim::box_free(d);
     void *ptrs[] = {d->g[0], d->g[1], d->f[0], d->f[1], d->d_off, d->d_node, d->d_type, d->d_perm, d->d_snode,
@@ -521,6 +529,9 @@ int casim_dyad_destroy(CasimDyad *d) {
     for (double *p : d->pool_buffers)
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
+    if (d->stream2) (void)hipStreamDestroy(d->stream2);
+    for (hipEvent_t e : {d->ev_fork, d->ev_off, d->ev_join})
+        if (e) (void)hipEventDestroy(e);
     delete d;
     return CASIM_OK;
 }
@@ -799,10 +810,44 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
         auto it = d->graphs.find(key);
         if (it == d->graphs.end()) {
             HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+            const bool pool_rim = d->Q.on && d->Q.cyto;
+            const bool pairs = d->pair_ok();
             try {
-                const bool pool_rim = d->Q.on && d->Q.cyto;
-                const bool pairs = d->pair_ok();
-                for (int k = 0; k < k_steps;) {
+                int k = 0;
+                if (d->split && pairs && !d->box && !pool_rim && d->P.S >= 2 && k_steps >= 2) {
+                    // Split pipelining: the two halves of the batch step on two
+                    // streams; the second starts one channel call after the
+                    // first, so one half's gating runs beside the other half's
+                    // pass.  Sites are independent, so the results are those of
+                    // the whole batch on one stream.
+                    const int npairs = k_steps / 2, half = (d->P.S + 1) / 2, cur_start = d->cur;
+                    HIPCK(hipEventRecord(d->ev_fork, d->stream));
+                    HIPCK(hipStreamWaitEvent(d->stream2, d->ev_fork, 0));
+                    for (int h = 0; h < 2; ++h) {
+                        d->range_s0 = h ? half : 0;
+                        d->range_n = h ? d->P.S - half : half;
+                        d->range_stream = h ? d->stream2 : d->stream;
+                        d->cur = cur_start;
+                        if (h) HIPCK(hipStreamWaitEvent(d->stream2, d->ev_off, 0));
+                        for (int i = 0; i < npairs; ++i) {
+                            d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (i ? CH_ACCUMULATE : 0) |
+                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                               dt);
+                            if (!h && !i) HIPCK(hipEventRecord(d->ev_off, d->stream));
+                            d->launch_pair(dt);
+                            d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE |
+                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                               dt);
+                        }
+                    }
+                    d->range_s0 = 0;
+                    d->range_n = -1;
+                    d->range_stream = nullptr;
+                    HIPCK(hipEventRecord(d->ev_join, d->stream2));
+                    HIPCK(hipStreamWaitEvent(d->stream, d->ev_join, 0));
+                    k = 2 * npairs;
+                }
+                for (; k < k_steps;) {
                     if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux
                         d->launch_rim(d->Q.cions, d->Q.cbb);
                         d->launch_efflux();
@@ -835,6 +880,14 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
                 d->launch_efflux();
                 d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, dt);
             } catch (...) {
+                // rejoin the second stream (if it was forked) so the capture can end
+                if (d->range_n >= 0) {
+                    (void)hipEventRecord(d->ev_join, d->stream2);
+                    (void)hipStreamWaitEvent(d->stream, d->ev_join, 0);
+                }
+                d->range_s0 = 0;
+                d->range_n = -1;
+                d->range_stream = nullptr;
                 hipGraph_t junk;
                 (void)hipStreamEndCapture(d->stream, &junk);
                 d->cur = cur0;
@@ -1227,6 +1280,17 @@ int casim_dyad_set_steps_per_pass(CasimDyad *d, int32_t steps) {
 }
 
 int32_t casim_dyad_steps_per_pass(const CasimDyad *d) { return d ? (d->pair_ok() ? 2 : 1) : -1; }
+
+int casim_dyad_set_split(CasimDyad *d, int32_t on) {
+    return guard([&] {
+        REQUIRE(d, CASIM_E_INVALID, "null handle");
+        REQUIRE(on == 0 || on == 1, CASIM_E_INVALID, "on must be 0 or 1");
+        d->use();
+        HIPCK(hipStreamSynchronize(d->stream));
+        if (d->split != (on != 0)) d->clear_graphs();
+        d->split = on != 0;
+    });
+}
 
 void *casim_dyad_stream(CasimDyad *d) { return d ? (void *)d->stream : nullptr; }
 

@@ -92,6 +92,17 @@ struct CasimDyad {
     int steps_per_pass = 2;   // 2: pairs of steps in one k_sweep2 pass where pair_ok()
     bool chan_small = false;  // the channel kernel's SheepCannell / one-ion instantiation applies
     casim::CytoBox *box = nullptr;  // cytosol box (null when not attached)
+    // the sites and stream the pair and channel launchers work on (all sites
+    // on `stream` unless casim_dyad_step splits the batch)
+    int range_s0 = 0, range_n = -1;
+    hipStream_t range_stream = nullptr;
+    // split pipelining: the batch's two halves on `stream` and `stream2`,
+    // offset by one channel call, so one half's gating overlaps the other's pass
+    bool split = true;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_off = nullptr, ev_join = nullptr;
+    int range_count() const { return range_n < 0 ? P.S : range_n; }
+    hipStream_t range_st() const { return range_stream ? range_stream : stream; }
     // captured K-step graphs, keyed by (k_steps, dt, state buffers in use:
     // bit 0 the dyad's, bit 1 the box's), and the buffer flips each makes
     std::map<std::tuple<int, double, int>, std::pair<hipGraphExec_t, int>> graphs;

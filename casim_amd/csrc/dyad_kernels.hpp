@@ -757,7 +757,7 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
                                                 const int *__restrict__ ch_off, const int *__restrict__ snode,
                                                 const double *__restrict__ sJ, const double *__restrict__ jnew,
                                                 T *__restrict__ ring, const Channels a, Params p, double dt_,
-                                                int chunk, int nchunks, int units) {
+                                                int chunk, int nchunks, int units, int s0) {
     constexpr int LI = NI ? NI : MAXI;
     constexpr int LB = NB ? NB : MAXB;
     constexpr int NG = NGT ? NGT : LI + LB;
@@ -765,11 +765,12 @@ __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__res
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
 
+    // units cover the sites s0 .. s0 + units / nchunks - 1
     const int per = gridDim.x >> 3;
     const int unit = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (unit >= units) return;
-    const int s = unit / nchunks;
-    const int cidx = unit - s * nchunks;
+    const int s = s0 + unit / nchunks;
+    const int cidx = unit - (s - s0) * nchunks;
     const int x0 = 1 + cidx * chunk;
     const int x1 = min(p.nx - 1, x0 + chunk);  // step k+2 planes [x0, x1)
     const int xa0 = x0 - 1;                    // step k+1 planes [xa0, x1]
@@ -1096,11 +1097,11 @@ __device__ __forceinline__ double draw_at(const uint32_t *__restrict__ jump, con
 // re-evaluated with the extra draw accounted for, until a ballot is empty.
 template <typename T, int MODEL, int NI_, int NB_, int NU_>
 __global__ __launch_bounds__(64) void k_channels(const T *__restrict__ g, const T *__restrict__ f, Channels a,
-                                                 Params p, Pools q, int mode, double dt) {
+                                                 Params p, Pools q, int mode, double dt, int s0) {
     // compile-time species counts where the layout is known (0: up to the maxima, guarded)
     constexpr int LI = NI_ ? NI_ : MAXI, LB = NB_ ? NB_ : MAXB, LU = NU_ ? NU_ : MAXU;
     __shared__ double Jl[LI][64];
-    const int s = blockIdx.x;
+    const int s = s0 + blockIdx.x;  // one workgroup per site of the range s0 ..
     const int lane = threadIdx.x;
     const int off = a.off[s];
     const int n = a.off[s + 1] - off;

@@ -63,15 +63,15 @@ void sweep2_ring(CasimDyad *d, double dt) {
     const int P = p.ny * p.nz;
     const int chunk = d->sweep_chunk;
     const int nchunks = (p.nx - 2 + chunk - 1) / chunk;
-    const int units = p.S * nchunks;
+    const int units = d->range_count() * nchunks;  // the launch range's sites
     const int grid = (units + 7) / 8 * 8;
     const size_t lds = (DIM == 3 ? 4 : 2) * (size_t)p.n_grid * P * sizeof(T);  // k_sweep2's LDS planes
     hipLaunchKernelGGL((k_sweep2<T, DIM, NI, NB, NG, NF, LY, SL, RING>), dim3(grid), dim3(d->sweep_threads), lds,
-                       d->stream,
+                       d->range_st(),
                        (const T *)d->g[d->cur], (T *)d->g[d->cur ^ 1], (const T *)d->f[d->cur],
                        (T *)d->f[d->cur ^ 1], d->d_off, d->d_snode, d->d_sJ,
                        (const double *)(d->Q.on ? d->Q.jnew : nullptr), (T *)box_ring(d), d->Ch, d->P, dt, chunk,
-                       nchunks, units);
+                       nchunks, units, d->range_s0);
 }
 
 // The ring store (step-k+1 values next to the rim, for a cytosol box) is a

@@ -213,5 +213,10 @@ class DyadHIP:
     def StepsPerPass(self):
         return self.lib.casim_dyad_steps_per_pass(self.h)
 
+    def SetSplit(self, on):
+        """Split pipelining of Step (two half-batches on two streams); the
+        results do not depend on it."""
+        abi.check(self.lib.casim_dyad_set_split(self.h, int(bool(on))))
+
     def stream(self):
         return self.lib.casim_dyad_stream(self.h)

@@ -444,3 +444,48 @@ def test_channel_count_edges(gpu, oracle_mod):
     assert np.array_equal(dev.GetChannelsStates(), ora.channel_states())
     assert np.array_equal(dev.GetRngStates(), ora.rng_states())
     compare(ora, dev, sp, FP64_TOL)
+
+
+@pytest.mark.parametrize("n_sites,k", [(5, 40), (64, 41)])
+def test_split_pipelining_equals_one_stream(gpu, n_sites, k):
+    """Step's split pipelining (two half-batches on two streams, the second one
+    channel call behind) gives the one-stream results bit for bit: fields,
+    faces, channel and RNG states, mouths, currents, release statistics and
+    open-count bins; an odd step count leaves one whole-batch step."""
+    sp = workloads.batched_dyads(n_sites=n_sites, geometry_seed=13)
+    out = []
+    for split in (0, 1):
+        dev = DyadHIP(sp)
+        dev.SetSplit(split)
+        dev.RecordOpenCounts(4, 30)
+        dev.InitOpeningAll()
+        ri, rb = sp.rim_arrays()
+        dev.Update(ri, rb, sp.jsr_array([1000.0]))
+        for _ in range(2):  # two graph launches compose
+            dev.Step(1e-5, k)
+        out.append([dev.GetField(f) for f in range(sp.n_ions + sp.n_bound)] +
+                   [dev.GetChannelsStates(), dev.GetRngStates(), dev.GetIonsNearChannels(),
+                    dev.GetTotalSRCurrent(), dev.GetReleaseStats(), dev.GetOpenCounts()])
+        dev.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    assert (out[1][sp.n_ions + sp.n_bound] == 1).sum() > 0  # channels open: the runs are not trivial
+
+
+def test_split_pipelining_with_a_jsr_pool(gpu):
+    """The same with a depleting jSR pool (per-site pool state on both streams)."""
+    from casim_amd import spark_compare
+    sp = workloads.batched_dyads(n_sites=33, geometry_seed=17)
+    out = []
+    for split in (0, 1):
+        dev = DyadHIP(sp)
+        dev.SetSplit(split)
+        dev.AttachPools(spark_compare.depleting_pool())
+        dev.InitOpeningAll()
+        dev.UpdatePools(*sp.rim_arrays())
+        dev.Step(1e-5, 60)
+        out.append([dev.GetField(0), dev.GetChannelsStates(), dev.GetRngStates(), dev.GetPools()[0][0],
+                    dev.GetReleaseStats()])
+        dev.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
