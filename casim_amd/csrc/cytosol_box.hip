@@ -118,10 +118,11 @@ __global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs 
 // the same for both steps.  The sums are acc + dt e(k), then + dt e(k+1), the
 // order of two separate calls.
 template <typename T>
-__global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring, BoxArgs a, double dt, int first) {
+__global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring, BoxArgs a, double dt, int first,
+                             int s0) {
 #pragma clang fp contract(off)
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
+    const int s = s0 + blockIdx.y;  // sites s0 .. s0 + gridDim.y - 1
     if (l >= a.nlink) return;
     const size_t in = (size_t)s * a.dNS + a.link_in[l];
     const size_t rin = (size_t)s * a.RN + (size_t)a.link_ring[l] * a.dnz;
@@ -314,13 +315,13 @@ T *upload(CytoBox *b, const std::vector<T> &h, hipStream_t st) {
 namespace {
 void efflux_launch(CasimDyad *d, const void *g, const void *ring, bool first, double dt) {
     CytoBox *b = d->box;
-    dim3 grid((b->a.nlink + 127) / 128, d->P.S);
+    dim3 grid((b->a.nlink + 127) / 128, d->range_count());  // the launch range's sites
     if (d->precision == CASIM_FP64)
-        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->stream, (const double *)g,
-                           (const double *)ring, b->a, dt, (int)first);
+        hipLaunchKernelGGL(k_box_efflux<double>, grid, dim3(128), 0, d->range_st(), (const double *)g,
+                           (const double *)ring, b->a, dt, (int)first, d->range_s0);
     else
-        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->stream, (const float *)g,
-                           (const float *)ring, b->a, dt, (int)first);
+        hipLaunchKernelGGL(k_box_efflux<float>, grid, dim3(128), 0, d->range_st(), (const float *)g,
+                           (const float *)ring, b->a, dt, (int)first, d->range_s0);
     HIPCK(hipGetLastError());
 }
 }  // namespace

@@ -330,11 +330,12 @@ int32_t casim_dyad_steps_per_pass(const CasimDyad *d);
 
 /* Split pipelining of casim_dyad_step (no counterpart in the reference, whose
  * batch loop steps every site in turn): with on = 1 (the default), pairs of
- * steps of a batch of two or more sites without a cytosol pool or box run as
- * two half-batches on two streams inside the captured graph, the second half
- * one channel call behind the first, so that one half's channel kernel runs
- * beside the other half's state pass.  Sites are independent, so the results
- * are the same bit for bit; 0 keeps one stream. */
+ * steps of a batch of two or more sites without a cytosol pool run as two
+ * half-batches on two streams inside the captured graph, the second half one
+ * channel call behind the first, so that one half's channel kernel (and a
+ * cytosol box's rim effluxes) run beside the other half's state pass; the
+ * halves rejoin for each box step.  Sites are independent, so the results are
+ * the same bit for bit; 0 keeps one stream. */
 int casim_dyad_set_split(CasimDyad *d, int32_t on);
 
 /*

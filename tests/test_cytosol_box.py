@@ -455,3 +455,36 @@ def test_gpu_box_profiled_steps_equal_graph_steps(gpu):
         for f in range(a.CytosolBoxFields()):
             assert np.array_equal(a.GetCytosolBox(f), b.GetCytosolBox(f)), (substeps, f)
         assert np.array_equal(a.GetChannelsStates(), b.GetChannelsStates())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gpu_box_split_pipelining_equals_one_stream(gpu, dim):
+    """Split pipelining with a cytosol box (halves rejoin for every box step)
+    gives the one-stream results bit for bit: dyad and box fields, channels,
+    RNG, jSR and release statistics."""
+    from casim_amd.dyad import DyadHIP
+    sp, box = small_case(dim=dim, n_sites=5, clamped=False, far=True, substeps=4)
+    sp.channel_types[0].params["kopen0"] *= 1e5
+    devs = []
+    for split in (0, 1):
+        d = DyadHIP(sp)
+        d.SetSplit(split)
+        d.AttachPools(depleting_pool())
+        d.AttachCytosolBox(box)
+        perturb(np.random.default_rng(11), sp, box, d.SetField, d.SetCytosolBox, d.GetCytosolBox)
+        d.InitOpeningAll()
+        d.UpdatePools()
+        for k in (4, 8, 12):
+            d.Step(2e-6, k)
+        assert d.StepsPerPass() == 2
+        devs.append(d)
+    one, two = devs
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(one.GetField(f), two.GetField(f)), f
+    for f in range(one.CytosolBoxFields()):
+        assert np.array_equal(one.GetCytosolBox(f), two.GetCytosolBox(f)), f
+    assert np.array_equal(one.GetChannelsStates(), two.GetChannelsStates())
+    assert np.array_equal(one.GetRngStates(), two.GetRngStates())
+    assert np.array_equal(one.GetPools()[0], two.GetPools()[0])
+    assert np.array_equal(one.GetReleaseStats(), two.GetReleaseStats())
