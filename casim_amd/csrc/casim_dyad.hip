@@ -814,50 +814,47 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
             const bool pairs = d->pair_ok();
             try {
                 int k = 0;
-                if (d->split && pairs && !pool_rim && d->P.S >= 2) {
+                if (d->split && pairs && !pool_rim && d->P.S >= 2 && k_steps >= 2) {
                     // Split pipelining: the two halves of the batch step on two
                     // streams; the second starts one channel call after the
                     // first, so one half's gating runs beside the other half's
-                    // pass.  Sites are independent, so the results are those of
-                    // the whole batch on one stream.  With a cytosol box the
-                    // halves rejoin for every box step (it spans the batch).
+                    // pass.  Sites are independent (a cytosol box too: each
+                    // site's box takes only that site's effluxes), so the
+                    // results are those of the whole batch on one stream.
                     const int half = (d->P.S + 1) / 2;
-                    while (k + 1 < k_steps) {
-                        int seg = (k_steps - k) / 2;  // pairs in this segment
-                        if (d->box) seg = std::min(seg, (M - k % M) / 2);
-                        const int cur_start = d->cur;
-                        HIPCK(hipEventRecord(d->ev_fork, d->stream));
-                        HIPCK(hipStreamWaitEvent(d->stream2, d->ev_fork, 0));
-                        for (int h = 0; h < 2; ++h) {
-                            d->range_s0 = h ? half : 0;
-                            d->range_n = h ? d->P.S - half : half;
-                            d->range_stream = h ? d->stream2 : d->stream;
-                            d->cur = cur_start;
-                            if (h) HIPCK(hipStreamWaitEvent(d->stream2, d->ev_off, 0));
-                            for (int i = 0; i < seg; ++i) {
-                                const int ks = k + 2 * i;  // the pair's first step
-                                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (ks ? CH_ACCUMULATE : 0) |
-                                                       CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                                   dt);
-                                if (!h && !i) HIPCK(hipEventRecord(d->ev_off, d->stream));
-                                d->launch_pair(dt);
-                                if (d->box) casim::box_efflux_pair(d, ks % M == 0, dt);
-                                d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE |
-                                                       CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                                   dt);
+                    const int npairs = k_steps / 2, cur_start = d->cur, bcur_start = casim::box_cur(d);
+                    HIPCK(hipEventRecord(d->ev_fork, d->stream));
+                    HIPCK(hipStreamWaitEvent(d->stream2, d->ev_fork, 0));
+                    for (int h = 0; h < 2; ++h) {
+                        d->range_s0 = h ? half : 0;
+                        d->range_n = h ? d->P.S - half : half;
+                        d->range_stream = h ? d->stream2 : d->stream;
+                        d->cur = cur_start;
+                        casim::box_set_cur(d, bcur_start);
+                        if (h) HIPCK(hipStreamWaitEvent(d->stream2, d->ev_off, 0));
+                        for (int i = 0; i < npairs; ++i) {
+                            const int ks = 2 * i;  // the pair's first step
+                            d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (ks ? CH_ACCUMULATE : 0) |
+                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                               dt);
+                            if (!h && !i) HIPCK(hipEventRecord(d->ev_off, d->stream));
+                            d->launch_pair(dt);
+                            if (d->box) casim::box_efflux_pair(d, ks % M == 0, dt);
+                            d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE |
+                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                               dt);
+                            if (d->box && (ks + 2) % M == 0) {  // box step on the accumulated effluxes, then the rim
+                                casim::box_step(d, M * dt);
+                                casim::box_rim(d);
                             }
                         }
-                        d->range_s0 = 0;
-                        d->range_n = -1;
-                        d->range_stream = nullptr;
-                        HIPCK(hipEventRecord(d->ev_join, d->stream2));
-                        HIPCK(hipStreamWaitEvent(d->stream, d->ev_join, 0));
-                        k += 2 * seg;
-                        if (d->box && k % M == 0) {  // box step on the accumulated effluxes, then the rim
-                            casim::box_step(d, M * dt);
-                            casim::box_rim(d);
-                        }
                     }
+                    d->range_s0 = 0;
+                    d->range_n = -1;
+                    d->range_stream = nullptr;
+                    HIPCK(hipEventRecord(d->ev_join, d->stream2));
+                    HIPCK(hipStreamWaitEvent(d->stream, d->ev_join, 0));
+                    k = 2 * npairs;
                 }
                 for (; k < k_steps;) {
                     if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux

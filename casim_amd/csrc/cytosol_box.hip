@@ -92,10 +92,10 @@ __global__ void k_box_init(double *b0, double *b1, const double *__restrict__ in
 
 // Dyad rim nodes from the box: one thread per (rim column, site).
 template <typename T>
-__global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs a) {
+__global__ void k_box_rim(T *g0, T *g1, const double *__restrict__ box, BoxArgs a, int s0) {
 #pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
+    const int s = s0 + blockIdx.y;  // sites s0 .. s0 + gridDim.y - 1
     if (c >= a.ncol) return;
     const size_t per = (size_t)a.S * a.NB;
     const int *q = a.col_box + 4 * c;
@@ -166,15 +166,16 @@ __global__ void k_box_efflux(const T *__restrict__ g, const T *__restrict__ ring
 // every array index is a constant and the values stay in registers.
 template <int NI, int NB>
 __global__ __launch_bounds__(1024) void k_box_step(const double *__restrict__ bin, double *__restrict__ bout, BoxArgs a,
-                                                   double dt, int ty, int ntile) {
+                                                   double dt, int ty, int ntile, int s0, int count) {
 #pragma clang fp contract(off)
     constexpr int NT = NI + NB;
     extern __shared__ double blds[];  // [2][tile nodes][NT]
     const int per_xcd = gridDim.x >> 3;
     const int unit = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int s = unit / ntile;
-    if (s >= a.S) return;  // whole workgroup
-    const int y0 = (unit - s * ntile) * ty;
+    const int sl = unit / ntile;  // site within s0 .. s0 + count - 1
+    if (sl >= count) return;      // whole workgroup
+    const int s = s0 + sl;
+    const int y0 = (unit - sl * ntile) * ty;
     const int TN = (ty + 2) * a.nx;
     const int t = threadIdx.x;
     const int x = t % a.nx, r = t / a.nx;
@@ -346,29 +347,29 @@ void box_step(CasimDyad *d, double dt) {
     const int ntile = (a.ny + ty - 1) / ty;
     const int threads = ((ty + 2) * a.nx + 63) / 64 * 64;
     const size_t lds = 2 * sizeof(double) * nt * (ty + 2) * a.nx;
-    const long units = (long)ntile * d->P.S;
+    const long units = (long)ntile * d->range_count();  // the launch range's sites
     REQUIRE(units < (1L << 31) - 8, CASIM_E_INVALID, "cytosol box batch too large for one launch");
     REQUIRE(threads <= 1024 && lds <= 65536, CASIM_E_INVALID, "cytosol box tile does not fit a workgroup");
     dim3 grid((unsigned)((units + 7) / 8 * 8));
     if (small)
-        hipLaunchKernelGGL((k_box_step<1, 3>), grid, dim3(threads), lds, d->stream, b->b[b->cur], b->b[b->cur ^ 1], a,
-                           dt, ty, ntile);
+        hipLaunchKernelGGL((k_box_step<1, 3>), grid, dim3(threads), lds, d->range_st(), b->b[b->cur],
+                           b->b[b->cur ^ 1], a, dt, ty, ntile, d->range_s0, d->range_count());
     else
-        hipLaunchKernelGGL((k_box_step<MAXI, MAXB>), grid, dim3(threads), lds, d->stream, b->b[b->cur],
-                           b->b[b->cur ^ 1], a, dt, ty, ntile);
+        hipLaunchKernelGGL((k_box_step<MAXI, MAXB>), grid, dim3(threads), lds, d->range_st(), b->b[b->cur],
+                           b->b[b->cur ^ 1], a, dt, ty, ntile, d->range_s0, d->range_count());
     HIPCK(hipGetLastError());
     b->cur ^= 1;
 }
 
 void box_rim(CasimDyad *d) {
     CytoBox *b = d->box;
-    dim3 grid((b->a.ncol + 127) / 128, d->P.S);
+    dim3 grid((b->a.ncol + 127) / 128, d->range_count());  // the launch range's sites
     if (d->precision == CASIM_FP64)
-        hipLaunchKernelGGL(k_box_rim<double>, grid, dim3(128), 0, d->stream, (double *)d->g[0], (double *)d->g[1],
-                           b->b[b->cur], b->a);
+        hipLaunchKernelGGL(k_box_rim<double>, grid, dim3(128), 0, d->range_st(), (double *)d->g[0],
+                           (double *)d->g[1], b->b[b->cur], b->a, d->range_s0);
     else
-        hipLaunchKernelGGL(k_box_rim<float>, grid, dim3(128), 0, d->stream, (float *)d->g[0], (float *)d->g[1],
-                           b->b[b->cur], b->a);
+        hipLaunchKernelGGL(k_box_rim<float>, grid, dim3(128), 0, d->range_st(), (float *)d->g[0], (float *)d->g[1],
+                           b->b[b->cur], b->a, d->range_s0);
     HIPCK(hipGetLastError());
 }
 
