@@ -135,6 +135,96 @@ __global__ void k_bias_act(float *__restrict__ Z, const float *__restrict__ b, u
     if (H) H[i] = f2bf(act ? z * sigmoidf_(z) : z);
 }
 
+// Inference of one ensemble member for IMR rows per workgroup, every layer in
+// one launch: the normalised BF16 input (k_gather's arithmetic), each hidden
+// layer as BF16 MFMA products with FP32 accumulation (k_gemm's operand layout
+// and k order), bias, SiLU and BF16 rounding (k_bias_act's), and the linear
+// output, with the activations in LDS.  The weights come transposed and
+// zero-padded ([n][k], k_wt) so a lane's eight k-values of an MFMA operand are
+// one 16-byte load.  Per output element the operations and their order are
+// those of k_gather + k_gemm + k_bias_act, so the outputs are the same bits.
+// Grid: (row tiles, members); four waves, wave w computes columns 64w .. +63.
+constexpr int IMR = 32, IMAXW = 256, IPADK = 8, IMAXM = 8, IMAXL = 8;
+struct InferArgs {
+    const uint16_t *wt[IMAXM];  // member's transposed weights
+    const float *p[IMAXM];      // member's FP32 parameters (biases at boff)
+    int wtoff[IMAXL], boff[IMAXL], kp[IMAXL], fout[IMAXL];
+    int nl, n_in;
+    const float *mean, *istd;
+};
+
+__global__ void k_wt(const uint16_t *__restrict__ W, uint16_t *__restrict__ WT, int fin, int fout, int kp, int np) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)np * kp) return;
+    const int n = (int)(i / kp), k = (int)(i % kp);
+    WT[i] = (n < fout && k < fin) ? W[(size_t)k * fout + n] : (uint16_t)0;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ X, int rows, float *__restrict__ out,
+                                                   int ostride, InferArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t H[2][IMR][IMAXW + IPADK];
+    const int m0 = blockIdx.x * IMR, e = blockIdx.y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int kp0 = a.kp[0];
+    for (int i = t; i < IMR * kp0; i += 256) {
+        const int r = i / kp0, f = i - r * kp0;
+        uint16_t v = 0;
+        if (m0 + r < rows && f < a.n_in) v = f2bf((X[(size_t)(m0 + r) * a.n_in + f] - a.mean[f]) * a.istd[f]);
+        H[0][r][f] = v;
+    }
+    __syncthreads();
+    const uint16_t *wt = a.wt[e];
+    const float *pe = a.p[e];
+    for (int L = 0; L < a.nl; ++L) {
+        const int src = L & 1, K = a.kp[L], N = a.fout[L], NP = (N + 15) & ~15;
+        const bool last = L == a.nl - 1;
+        const int n0 = w * 64;
+        if (n0 < NP) {
+            const int nj = min(4, (NP - n0) >> 4);
+            f32x4_t acc[2][4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            const uint16_t *wl = wt + a.wtoff[L];
+            for (int k0 = 0; k0 < K; k0 += 32) {
+                bf16x8_t av[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    av[i] = *reinterpret_cast<const bf16x8_t *>(&H[src][i * 16 + (l & 15)][k0 + (l >> 4) * 8]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j >= nj) break;
+                    const bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(
+                        wl + (size_t)(n0 + j * 16 + (l & 15)) * K + k0 + (l >> 4) * 8);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[i][j], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j >= nj) break;
+                const int col = n0 + j * 16 + (l & 15);
+                if (col >= N) continue;
+                const float bias = pe[a.boff[L] + col];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = i * 16 + (l >> 4) * 4 + r;
+                        const float z = acc[i][j][r] + bias;
+                        if (last) {
+                            if (m0 + row < rows) out[(size_t)e * ostride + m0 + row] = z;
+                        } else {
+                            H[src ^ 1][row][col] = f2bf(z * sigmoidf_(z));
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // dZ = dH * SiLU'(Z), kept in FP32 (bias gradient) and BF16 (GEMM operand).
 __global__ void k_silu_back(const float *__restrict__ dH, const float *__restrict__ Z, float *__restrict__ dZ,
                             uint16_t *__restrict__ dZb, size_t n) {
@@ -218,6 +308,15 @@ struct CasimMLP {
     float *X = nullptr, *Y = nullptr;
     int *perm_d = nullptr;  // [rows] shuffled row order of the current epoch
     int64_t rows = 0;
+    // fused inference (k_mlp_infer): transposed weights, refreshed when the
+    // parameters change (pver counts parameter updates, wt_ver the copy's)
+    bool fused = true;
+    uint16_t *wt = nullptr;  // [members][wt_per]
+    size_t wt_per = 0;
+    uint64_t pver = 1, wt_ver = 0;
+    bool fused_ok() const;
+    void refresh_wt();
+    void infer(const float *x, int B, float *out, int ostride);
     // batch workspace
     int cap = 0;
     std::vector<uint16_t *> Xl;  // [depth + 1] BF16 activations
@@ -239,6 +338,58 @@ void CasimMLP::gemm(const uint16_t *A, int lda, int ta, const uint16_t *B, int l
                     int N, int K, float beta) {
     dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM);
     hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, stream, A, lda, ta, B, ldb, tb, C, ldc, M, N, K, beta);
+    HIPCK(hipGetLastError());
+}
+
+bool CasimMLP::fused_ok() const {
+    return fused && members <= IMAXM && depth + 1 <= IMAXL && n_in <= IMAXW && width <= IMAXW && width % 64 == 0;
+}
+
+void CasimMLP::refresh_wt() {
+    if (wt && wt_ver == pver) return;
+    std::vector<size_t> off;
+    size_t per = 0;
+    for (int l = 0; l <= depth; ++l) {
+        off.push_back(per);
+        per += (size_t)((fout[l] + 15) & ~15) * ((fin[l] + 31) & ~31);
+    }
+    if (!wt) {
+        HIPCK(hipMalloc(&wt, sizeof(uint16_t) * per * members));
+        wt_per = per;
+    }
+    for (int e = 0; e < members; ++e)
+        for (int l = 0; l <= depth; ++l) {
+            const int kp = (fin[l] + 31) & ~31, np = (fout[l] + 15) & ~15;
+            const size_t n = (size_t)np * kp;
+            hipLaunchKernelGGL(k_wt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, pb[e] + woff[l],
+                               wt + (size_t)e * wt_per + off[l], fin[l], fout[l], kp, np);
+            HIPCK(hipGetLastError());
+        }
+    wt_ver = pver;
+}
+
+// Every member's output (normalised units) for B raw rows x -> out[e * ostride + row].
+void CasimMLP::infer(const float *x, int B, float *out, int ostride) {
+    refresh_wt();
+    InferArgs a{};
+    size_t off = 0;
+    for (int l = 0; l <= depth; ++l) {
+        a.wtoff[l] = (int)off;
+        a.kp[l] = (fin[l] + 31) & ~31;
+        a.fout[l] = fout[l];
+        a.boff[l] = (int)boff[l];
+        off += (size_t)((fout[l] + 15) & ~15) * a.kp[l];
+    }
+    for (int e = 0; e < members; ++e) {
+        a.wt[e] = wt + (size_t)e * wt_per;
+        a.p[e] = p[e];
+    }
+    a.nl = depth + 1;
+    a.n_in = n_in;
+    a.mean = mean;
+    a.istd = istd;
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((B + IMR - 1) / IMR), (unsigned)members), dim3(256), 0, stream, x,
+                       B, out, ostride, a);
     HIPCK(hipGetLastError());
 }
 
@@ -317,9 +468,13 @@ void mlp_forward_device(CasimMLP *m, const float *x, int64_t rows, float *out, h
     m->stream = stream;
     try {
         const int chunk = (int)std::min<int64_t>(rows, 65536);
-        m->reserve(chunk);
+        if (!m->fused_ok()) m->reserve(chunk);
         for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
             const int B = (int)std::min<int64_t>(chunk, rows - r0);
+            if (m->fused_ok()) {
+                m->infer(x + r0 * m->n_in, B, out + r0, (int)rows);
+                continue;
+            }
             const size_t n = (size_t)B * m->n_in;
             hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                                x + r0 * m->n_in, (const float *)nullptr, (const int *)nullptr, B, m->n_in, m->mean,
@@ -494,6 +649,7 @@ int casim_mlp_destroy(CasimMLP *m) {
     for (auto v : {&m->p, &m->g, &m->am, &m->av})
         for (float *q : *v) (void)hipFree(q);
     for (uint16_t *q : m->pb) (void)hipFree(q);
+    if (m->wt) (void)hipFree(m->wt);
     for (void *q : {(void *)m->mean, (void *)m->istd, (void *)m->X, (void *)m->Y, (void *)m->perm_d})
         if (q) (void)hipFree(q);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -523,6 +679,7 @@ int casim_mlp_set_params(CasimMLP *m, int32_t member, const float *in) {
                            m->pb[member], m->P);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(m->stream));
+        ++m->pver;
     });
 }
 
@@ -573,6 +730,7 @@ int casim_mlp_train(CasimMLP *m, int32_t epochs, int32_t batch, float lr, uint64
         float *sq_h = nullptr;
         std::vector<float> sq(B);
         (void)sq_h;
+        ++m->pver;  // the transposed inference weights follow the parameters
         for (int e = 0; e < m->members; ++e) {
             std::mt19937_64 gen(seed * 1315423911ull + (uint64_t)e * 2654435761ull + 17);
             double last = 0.0;
@@ -654,15 +812,27 @@ int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, fl
         REQUIRE(m && x && rows >= 1, CASIM_E_INVALID, "bad arguments");
         m->use();
         const int chunk = (int)std::min<int64_t>(rows, 65536);
-        m->reserve(chunk);
-        float *dx = nullptr;
+        const bool fused = m->fused_ok();
+        if (!fused) m->reserve(chunk);
+        float *dx = nullptr, *dout = nullptr;
         HIPCK(hipMalloc(&dx, sizeof(float) * (size_t)chunk * m->n_in));
+        if (fused) HIPCK(hipMalloc(&dout, sizeof(float) * (size_t)chunk * m->members));
         std::vector<float> out((size_t)m->members * rows);
         std::vector<float> tmp(chunk);
         for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
             const int B = (int)std::min<int64_t>(chunk, rows - r0);
             HIPCK(hipMemcpyAsync(dx, x + r0 * m->n_in, sizeof(float) * (size_t)B * m->n_in, hipMemcpyHostToDevice,
                                  m->stream));
+            if (fused) {
+                m->infer(dx, B, dout, B);
+                for (int e = 0; e < m->members; ++e) {
+                    HIPCK(hipMemcpyAsync(tmp.data(), dout + (size_t)e * B, sizeof(float) * B, hipMemcpyDeviceToHost,
+                                         m->stream));
+                    HIPCK(hipStreamSynchronize(m->stream));
+                    for (int i = 0; i < B; ++i) out[(size_t)e * rows + r0 + i] = tmp[i] * m->ys + m->ym;
+                }
+                continue;
+            }
             const size_t n = (size_t)B * m->n_in;
             hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, dx,
                                (const float *)nullptr, (const int *)nullptr, B, m->n_in, m->mean, m->istd, 0.f, 1.f,
@@ -677,6 +847,7 @@ int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, fl
             }
         }
         (void)hipFree(dx);
+        if (dout) (void)hipFree(dout);
         for (int64_t i = 0; i < rows; ++i) {
             double s = 0, s2 = 0;
             for (int e = 0; e < m->members; ++e) {
@@ -690,6 +861,14 @@ int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, fl
             if (spread) spread[i] = (float)(mu > 0 ? std::sqrt(var) / mu : 0.0);
         }
         if (member_out) std::memcpy(member_out, out.data(), sizeof(float) * out.size());
+    });
+}
+
+int casim_mlp_set_fused_inference(CasimMLP *m, int32_t on) {
+    return guard([&] {
+        REQUIRE(m, CASIM_E_INVALID, "null handle");
+        REQUIRE(on == 0 || on == 1, CASIM_E_INVALID, "on must be 0 or 1");
+        m->fused = on != 0;
     });
 }
 

@@ -22,6 +22,7 @@ def _declare(lib):
         "casim_mlp_train": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
         "casim_mlp_gradient": (C.c_int, [P, C.c_int32, _fp, _fp, C.c_int64, _fp, _fp]),
         "casim_mlp_predict": (C.c_int, [P, _fp, C.c_int64, _fp, _fp, _fp]),
+        "casim_mlp_set_fused_inference": (C.c_int, [P, C.c_int32]),
         "casim_qs_create": (C.c_int, [P, C.POINTER(P)]),
         "casim_qs_destroy": (C.c_int, [P]),
         "casim_qs_reset": (C.c_int, [P]),
@@ -115,6 +116,10 @@ class MLP:
         loss = np.zeros(1, dtype=np.float32)
         abi.check(self.lib.casim_mlp_gradient(self.h, member, fptr(x), fptr(y), len(y), fptr(g), fptr(loss)))
         return float(loss[0]), g
+
+    def set_fused(self, on):
+        """Fused inference kernel (default) or the per-layer GEMMs; same outputs."""
+        abi.check(self.lib.casim_mlp_set_fused_inference(self.h, int(bool(on))))
 
     def predict(self, x):
         """(ensemble mean of exp(output), relative spread, per-member raw outputs)."""

@@ -68,6 +68,15 @@ int casim_mlp_gradient(CasimMLP *m, int32_t member, const float *x, const float 
  * member_out [members x rows] receives every member's raw output. */
 int casim_mlp_predict(CasimMLP *m, const float *x, int64_t rows, float *mean, float *spread, float *member_out);
 
+/* Inference kernel choice (no counterpart in the reference, which has no
+ * learned model): with on = 1 (the default) casim_mlp_predict and the
+ * quasi-stationary path run every layer of a member in one fused kernel
+ * (activations in LDS, BF16 MFMA, FP32 accumulation) where the shape allows
+ * it (members <= 8, layers <= 8, inputs and width <= 256, width a multiple
+ * of 64); 0 selects the per-layer GEMM and bias kernels.  The outputs are
+ * the same bits either way. */
+int casim_mlp_set_fused_inference(CasimMLP *m, int32_t on);
+
 /*
  * Ensemble prediction that allocates nothing (no reference counterpart): for
  * callers that cannot allocate device memory, such as code under HIP graph

@@ -97,3 +97,40 @@ def test_training_reduces_loss(gpu):
     mean, spread, raw = m.predict(x[:512])
     pred = np.log(mean)
     assert np.sqrt(np.mean((raw.mean(axis=0) - y[:512]) ** 2)) < 0.25 * np.std(y)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_in,width,depth,members,rows", [(57, 256, 4, 5, 1000), (57, 256, 4, 5, 33),
+                                                          (10, 128, 2, 3, 1), (200, 64, 3, 8, 97)])
+def test_fused_inference_equals_layer_by_layer(gpu, n_in, width, depth, members, rows):
+    """The fused inference kernel (every layer of a member in one launch,
+    activations in LDS) gives the per-layer GEMM + bias path's outputs bit for
+    bit, for the surrogate's shape, narrower layers, one row and ragged tiles;
+    and it follows parameter updates (set_params, training)."""
+    from casim_amd import mlp
+    gen = np.random.default_rng(n_in + width + rows)
+    net = mlp.MLP(n_in, width=width, depth=depth, members=members, seed=5)
+    x = gen.normal(0, 2, (rows, n_in)).astype(np.float32)
+    net.set_normalization(x.mean(axis=0), x.std(axis=0) + 0.5, 0.3, 1.7)
+    for e in range(members):
+        net.set_params(e, gen.normal(0, 0.08, net.P).astype(np.float32))
+
+    def both():
+        net.set_fused(0)
+        a = net.predict(x)
+        net.set_fused(1)
+        b = net.predict(x)
+        return a, b
+
+    a, b = both()
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+    assert np.all(np.isfinite(b[2])) and np.ptp(b[2]) > 0
+    # after a parameter update the fused path uses the new weights
+    net.set_params(0, gen.normal(0, 0.08, net.P).astype(np.float32))
+    a2, b2 = both()
+    assert np.array_equal(a2[2], b2[2]) and not np.array_equal(b2[2][0], b[2][0])
+    net.set_data(x, gen.normal(0, 1, rows).astype(np.float32))
+    net.train(1, batch=min(rows, 64), lr=1e-3, seed=3)
+    a3, b3 = both()
+    assert np.array_equal(a3[2], b3[2]) and not np.array_equal(b3[2], b2[2])
