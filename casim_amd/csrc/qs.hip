@@ -38,6 +38,7 @@ struct QsDev {
     int *active;   // 1 while the site evolves
     int *waiting;  // 0 no, 1 needs the full solver, 2 values supplied
     int *crow;     // [S] first surrogate row of the site this round, -1 if none
+    int *list;     // the sites with rows this round, in site order (k_qs_plan)
     uint8_t *trace;  // [bins][S] open channels per time bin (or null)
     int *cur_bin;    // [S] first bin not yet written
     double bin_ms;
@@ -112,7 +113,7 @@ __device__ double stationary_mouth(const QsDev &q, int off, int n, int i) {
 __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
     __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
     __shared__ int so[QS_MAXCH];
-    const int s = blockIdx.x;
+    const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
     const int base = q.crow[s];
     if (base < 0) return;
     const int off = q.off[s], n = q.off[s + 1] - off;
@@ -205,7 +206,7 @@ __device__ double jsr_advance(const QsDev &q, double c, double geff, double dt) 
 __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     __shared__ double rate[QS_MAXCH];
     __shared__ float spv[QS_MAXCH], cav[QS_MAXCH];
-    const int s = blockIdx.x;
+    const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
     const int lane = threadIdx.x;
     const int base = q.crow[s];
     if (base < 0) return;
@@ -376,34 +377,40 @@ __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t 
 // not wait for the full solver, packed in site order (one block).  cnt[0..2]:
 // active sites, waiting sites, rows.
 __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restrict__ cnt) {
-    __shared__ int part[1024];
+    __shared__ int part[1024], pn[1024];
     __shared__ int pa[1024], pw[1024];
     const int t = threadIdx.x;
     const int per = (q.S + 1023) / 1024;
     const int s0 = t * per, s1 = min(q.S, s0 + per);
-    int rows = 0, na = 0, nw = 0;
+    int rows = 0, na = 0, nw = 0, nc = 0;
     for (int s = s0; s < s1; ++s) {
         const bool act = q.active[s] != 0, w = act && q.waiting[s] == 1;
         na += act;
         nw += w;
-        if (all || (act && !w)) rows += q.off[s + 1] - q.off[s];
+        if (all || (act && !w)) {
+            rows += q.off[s + 1] - q.off[s];
+            ++nc;
+        }
     }
     part[t] = rows;
+    pn[t] = nc;
     pa[t] = na;
     pw[t] = nw;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? part[t - d] : 0;
+        const int v = t >= d ? part[t - d] : 0, vn = t >= d ? pn[t - d] : 0;
         __syncthreads();
         part[t] += v;
+        pn[t] += vn;
         __syncthreads();
     }
-    int r = part[t] - rows;
+    int r = part[t] - rows, c = pn[t] - nc;
     for (int s = s0; s < s1; ++s) {
         const bool act = q.active[s] != 0, w = act && q.waiting[s] == 1;
         if (all || (act && !w)) {
             q.crow[s] = r;
             r += q.off[s + 1] - q.off[s];
+            q.list[c++] = s;
         } else {
             q.crow[s] = -1;
         }
@@ -417,6 +424,7 @@ __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restr
         cnt[0] = ta;
         cnt[1] = tw;
         cnt[2] = part[1023];
+        cnt[3] = pn[1023];  // sites with rows: the grid of k_qs_features and k_qs_event
     }
 }
 
@@ -567,6 +575,7 @@ int casim_qs_create(const CasimQSConfig *c, CasimQS **out) {
         q.bins = 0;
         q.bin_ms = 0.01;
         q.crow = h->alloc<int>(S);
+        q.list = h->alloc<int>(S);
         h->d_rng0 = h->alloc<uint32_t>((size_t)S * 6);
         h->d_jump = h->alloc<uint32_t>(jump.size());
         HIPCK(hipMemcpyAsync(h->d_init, h->h_init.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
@@ -638,7 +647,7 @@ int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch
         auto plan = [&] {
             hipLaunchKernelGGL(k_qs_plan, dim3(1), dim3(1024), 0, h->stream, h->q, 0, h->d_count);
             HIPCK(hipGetLastError());
-            HIPCK(hipMemcpyAsync(cnt, h->d_count, sizeof(int) * 3, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(hipMemcpyAsync(cnt, h->d_count, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
             HIPCK(hipStreamSynchronize(h->stream));
         };
         plan();
@@ -646,11 +655,11 @@ int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch
         while (r < max_rounds && cnt[0] > 0 && cnt[1] < batch && cnt[1] < cnt[0]) {
             h->q.R = cnt[2];
             if (!stationary) {
-                hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+                hipLaunchKernelGGL(k_qs_features, dim3(cnt[3]), dim3(64), 0, h->stream, h->q);
                 HIPCK(hipGetLastError());
                 mlp_forward_device(m, h->q.X, cnt[2], h->q.out, h->stream);
             }
-            hipLaunchKernelGGL(k_qs_event, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+            hipLaunchKernelGGL(k_qs_event, dim3(cnt[3]), dim3(64), 0, h->stream, h->q);
             HIPCK(hipGetLastError());
             ++r;
             plan();
