@@ -112,19 +112,27 @@ __device__ double stationary_mouth(const QsDev &q, int off, int n, int i) {
 // like a stable argsort).
 __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
     __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
-    __shared__ int so[QS_MAXCH];
+    __shared__ int so[QS_MAXCH], ol[QS_MAXCH];
     const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
     const int base = q.crow[s];
     if (base < 0) return;
     const int off = q.off[s], n = q.off[s + 1] - off;
-    for (int i = threadIdx.x; i < n; i += 64) {
-        sx[i] = q.x[off + i];
-        sy[i] = q.y[off + i];
-        so[i] = q.state[off + i] == 1;
+    // the open channels in index order (a ballot per 64 channels), so that the
+    // sums below run over the open ones only, in the order of a full scan
+    int nopen = 0;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool op = i < n && q.state[off + i] == 1;
+        if (i < n) {
+            sx[i] = q.x[off + i];
+            sy[i] = q.y[off + i];
+            so[i] = op;
+        }
+        const unsigned long long m = __ballot(op);
+        if (op) ol[nopen + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = i;
+        nopen += __popcll(m);
     }
     __syncthreads();
-    int nopen = 0;
-    for (int j = 0; j < n; ++j) nopen += so[j];
     for (int i = threadIdx.x; i < n; i += 64) {
         float *f = q.X + (size_t)(base + i) * q.nf;
         const float xi = sx[i], yi = sy[i];
@@ -136,8 +144,9 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
             dyk[k] = 0.f;
         }
         float sinv = 0.f, sexp = 0.f, lin = 0.f;
-        for (int j = 0; j < n; ++j) {
-            if (!so[j] || j == i) continue;
+        for (int o = 0; o < nopen; ++o) {
+            const int j = ol[o];
+            if (j == i) continue;
             const float dx = sx[j] - xi, dy = sy[j] - yi;
             const float r2 = dx * dx + dy * dy;
             const float r = sqrtf(r2);
