@@ -335,6 +335,7 @@ int casim_dyad_create(const CasimDyadConfig *c, CasimDyad **out) {
         P.ihx2 = 1.0 / (c->dx * c->dx);
         P.ihy2 = 1.0 / (c->dy * c->dy);
         P.ihz2 = c->dim == 3 ? 1.0 / (c->dz * c->dz) : 0.0;
+        P.cc = 2.0 * (P.ihx2 + P.ihy2 + P.ihz2);
         d->elem_volume = c->dim == 3 ? 0.5 * c->dx * c->dy * c->dz : c->dx * c->dy * c->z;
         d->grad_mult = 3.0 / (4 * std::acos(-1.0) * c->radius * c->radius * c->radius);
         d->sxm = c->dim == 3 ? c->dy * c->dz / c->dx : c->dy * c->z / c->dx;

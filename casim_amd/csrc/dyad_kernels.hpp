@@ -40,6 +40,7 @@ struct Params {
     double c0[MAXI];
     double gD[MAXI + MAXB];  // diffusion coefficient of grid field g
     double ihx2, ihy2, ihz2;
+    double cc;  // 2 (ihx2 + ihy2 + ihz2): the centre weight of the Laplacian
 };
 
 struct Channels {
@@ -263,33 +264,35 @@ __global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__re
     T C[LI], evoC[LI], B[LB], evoB[LB];
     const bool face = DIM == 3 && (z == 0 || z == p.nz - 1);
     const int xy = x * p.ny + y;
+    const T cc = (T)p.cc;
 
+    // the arithmetic of node_update (neighbour pairs summed first, the
+    // diffusion coefficient applied inside the evolution)
 #pragma unroll
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
         const T *u = gin + j * fs + c;
         T vc = u[0];
         C[j] = vc;
-        T lap = (u[sx] - 2 * vc + u[-sx]) * ihx2 + (u[sy] - 2 * vc + u[-sy]) * ihy2;
-        if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
-        evoC[j] = (T)p.Dion[j] * lap;
+        T lap = (u[sx] + u[-sx]) * ihx2 + (u[sy] + u[-sy]) * ihy2;
+        if (DIM == 3) lap = lap + (u[zr] + u[zl]) * ihz2;
+        evoC[j] = lap - cc * vc;  // the Laplacian until the evolution below
     }
+    T lB[LB];
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
         if (!NB && b >= p.nb) break;
+        lB[b] = 0;
         if (p.bface[b]) {
             B[b] = face ? load_face(fin, p, p.bslot[b], s, z == 0 ? 0 : 1, xy) : (T)0;
-            evoB[b] = 0;
         } else {
             const T *u = gin + p.bslot[b] * fs + c;
             T vc = u[0];
             B[b] = vc;
             if (p.Db[b] != 0.0) {
-                T lap = (u[sx] - 2 * vc + u[-sx]) * ihx2 + (u[sy] - 2 * vc + u[-sy]) * ihy2;
-                if (DIM == 3) lap = lap + (u[zr] - 2 * vc + u[zl]) * ihz2;
-                evoB[b] = (T)p.Db[b] * lap;
-            } else {
-                evoB[b] = 0;
+                T lap = (u[sx] + u[-sx]) * ihx2 + (u[sy] + u[-sy]) * ihy2;
+                if (DIM == 3) lap = lap + (u[zr] + u[zl]) * ihz2;
+                lB[b] = lap - cc * vc;
             }
         }
     }
@@ -307,36 +310,42 @@ __global__ __launch_bounds__(256) void k_step(const T *__restrict__ gin, T *__re
         frb[b] = v;
     }
     // reactions R1 = koff B - kon C Bfree
+    T R[LI];
 #pragma unroll
-    for (int j = 0; j < LI; ++j) {
-        if (!NI && j >= p.ni) break;
-        T R = 0;
+    for (int j = 0; j < LI; ++j) R[j] = (T)-0.0;
 #pragma unroll
-        for (int b = 0; b < LB; ++b) {
-            if (!NB && b >= p.nb) break;
-            if (p.bion[b] != j) continue;
-            if (p.bface[b] && !face) continue;
-            T R1 = (T)p.koff[b] * B[b] - (T)p.kon[b] * C[j] * frb[b];
-            evoB[b] += -R1;
-            R += R1;
-        }
-        evoC[j] += R;
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        const int j = p.bion[b];
+        T cj = (T)0;
+#pragma unroll
+        for (int jj = 0; jj < LI; ++jj)
+            if (jj == j) cj = C[jj];
+        const T R1 = (T)p.koff[b] * B[b] - (T)p.kon[b] * cj * frb[b];
+        evoB[b] = p.Db[b] != 0.0 && !p.bface[b] ? (T)p.Db[b] * lB[b] - R1 : -R1;
+        const bool on = !p.bface[b] || face;
+#pragma unroll
+        for (int jj = 0; jj < LI; ++jj)
+            if (jj == j && on) R[jj] += R1;
     }
     // currents at mouth nodes (the last channel on a node wins, as the
     // reference's assignment does)
+    T Jn[LI];
+#pragma unroll
+    for (int j = 0; j < LI; ++j) Jn[j] = 0;
     for (int k = lo; k < hi; ++k) {
         if (snode[k] == n) {
 #pragma unroll
             for (int j = 0; j < LI; ++j) {
                 if (!NI && j >= p.ni) break;
-                evoC[j] = evoC[j] + (T)sJ[(size_t)j * nch + k];
+                Jn[j] = (T)sJ[(size_t)j * nch + k];
             }
         }
     }
 #pragma unroll
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
-        T e = evoC[j];
+        T e = (T)p.Dion[j] * evoC[j] + (R[j] + Jn[j]);
         if ((DIM == 2 || z == p.nz - 1) && p.sl[j]) {
             T s1 = (T)p.K1[j] + C[j];
             T s2 = (T)p.K2[j] + C[j];
@@ -433,7 +442,7 @@ template <int LY> __device__ __forceinline__ double ly_total(const Params &p, in
     return (faces && p.dim == 3 && z != 0 && z != p.nz - 1) ? 0.0 : p.total[u];
 }
 
-// Updated values of one interior node in DyadRDCPU.cpp:404-492 order: the
+// Updated values of one interior node (DyadRDCPU.cpp:404-492): the
 // Laplacian of every diffusing field (x-neighbours from registers, y and z
 // from the LDS plane), the reactions with the free buffers seen by each
 // bound field, the channel current, the sarcolemmal factor at z = nz-1 (3D)
@@ -443,6 +452,14 @@ template <int LY> __device__ __forceinline__ double ly_total(const Params &p, in
 // its y-neighbours sit sy elements away and its z-neighbours zl, zr elements
 // away (the mirror image on the z-faces).  Writes the new grid values to G
 // (every grid field) and, at face nodes, the new face values to Fo.
+//
+// The reference's terms, regrouped for FP64 issue: each Laplacian sums the
+// two neighbours of a direction first, ihx2 (u[x+1] + u[x-1]) + ihy2 (..)
+// + ihz2 (..) - cc u with cc = 2 (ihx2 + ihy2 + ihz2), and the diffusion
+// coefficient multiplies it inside the evolution, D lap + (reactions +
+// current) for an ion and D lap - R1 for a bound field (7 + 2 operations
+// per diffusing field where the reference's order takes 10 + 3).  The
+// values differ from the reference's order by rounding only.
 template <typename T, int DIM, int NI, int NB, int NG, int NF, int LY, int LI_ = NI ? NI : MAXI>
 __device__ __forceinline__ void node_update(const Params &p, const T *Cm, const T (&C0)[NG],
                                             const T (&Cp)[NG], const T *Lc, int sy, int zl, int zr, int z,
@@ -450,28 +467,29 @@ __device__ __forceinline__ void node_update(const Params &p, const T *Cm, const 
                                             T (&G)[NG], T (&Fo)[NF]) {
     constexpr int LI = NI ? NI : MAXI;
     constexpr int LB = NB ? NB : MAXB;
-    T Cv[LI], eC[LI], Bv[LB], eB[LB];
+    const T cc = (T)p.cc;
+    T Cv[LI], lC[LI], Bv[LB], lB[LB];
     const T *Lyl = Lc - sy, *Lyr = Lc + sy, *Lzl = Lc + zl, *Lzr = Lc + zr;
 #pragma unroll
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
         const T vc = C0[j];
         Cv[j] = vc;
-        T lap = (Cp[j] - 2 * vc + Cm[j]) * ihx2 + (Lyr[j] - 2 * vc + Lyl[j]) * ihy2;
-        if (DIM == 3) lap = lap + (Lzr[j] - 2 * vc + Lzl[j]) * ihz2;
-        eC[j] = (T)p.Dion[j] * lap;
+        T lap = (Cp[j] + Cm[j]) * ihx2 + (Lyr[j] + Lyl[j]) * ihy2;
+        if (DIM == 3) lap = lap + (Lzr[j] + Lzl[j]) * ihz2;
+        lC[j] = lap - cc * vc;
     }
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
         if (!NB && b >= p.nb) break;
         const int g = ly_bslot<LY>(p, b);
+        lB[b] = 0;
         if (ly_bface<LY>(p, b)) {
             T fv = (T)0;
 #pragma unroll
             for (int f = 0; f < NF; ++f)
                 if (f == g) fv = Fv[f];
             Bv[b] = face ? fv : (T)0;
-            eB[b] = 0;
         } else {
             T vc = (T)0, cxr = (T)0, cxl = (T)0;
 #pragma unroll
@@ -483,11 +501,9 @@ __device__ __forceinline__ void node_update(const Params &p, const T *Cm, const 
                 }
             Bv[b] = vc;
             if (ly_mobile<LY>(p, b)) {
-                T lap = (cxr - 2 * vc + cxl) * ihx2 + (Lyr[g] - 2 * vc + Lyl[g]) * ihy2;
-                if (DIM == 3) lap = lap + (Lzr[g] - 2 * vc + Lzl[g]) * ihz2;
-                eB[b] = (T)p.Db[b] * lap;
-            } else {
-                eB[b] = 0;
+                T lap = (cxr + cxl) * ihx2 + (Lyr[g] + Lyl[g]) * ihy2;
+                if (DIM == 3) lap = lap + (Lzr[g] + Lzl[g]) * ihz2;
+                lB[b] = lap - cc * vc;
             }
         }
     }
@@ -505,25 +521,31 @@ __device__ __forceinline__ void node_update(const Params &p, const T *Cm, const 
         }
         frb[b] = v;
     }
+    // R1 = koff B - kon C Bfree per bound field; the evolution of a bound
+    // field is D lap - R1 (-R1 when it does not diffuse), an ion's gains the
+    // sum of its fields' R1 (a z-face buffer's only at face nodes)
+    T eB[LB], Rr[LI];
 #pragma unroll
-    for (int j = 0; j < LI; ++j) {
-        if (!NI && j >= p.ni) break;
-        T Rr = 0;
+    for (int j = 0; j < LI; ++j) Rr[j] = (T)-0.0;  // -0 + x == x: the first term enters exactly
 #pragma unroll
-        for (int b = 0; b < LB; ++b) {
-            if (!NB && b >= p.nb) break;
-            if (ly_bion<LY>(p, b) != j) continue;
-            if (ly_bface<LY>(p, b) && !face) continue;
-            T R1 = (T)p.koff[b] * Bv[b] - (T)p.kon[b] * Cv[j] * frb[b];
-            eB[b] += -R1;
-            Rr += R1;
-        }
-        eC[j] += Rr;
+    for (int b = 0; b < LB; ++b) {
+        if (!NB && b >= p.nb) break;
+        const int j = ly_bion<LY>(p, b);
+        T cj = (T)0;
+#pragma unroll
+        for (int jj = 0; jj < LI; ++jj)
+            if (jj == j) cj = Cv[jj];
+        const T R1 = (T)p.koff[b] * Bv[b] - (T)p.kon[b] * cj * frb[b];
+        eB[b] = ly_mobile<LY>(p, b) ? (T)p.Db[b] * lB[b] - R1 : -R1;
+        const bool on = !ly_bface<LY>(p, b) || face;
+#pragma unroll
+        for (int jj = 0; jj < LI; ++jj)
+            if (jj == j && on) Rr[jj] += R1;
     }
 #pragma unroll
     for (int j = 0; j < LI; ++j) {
         if (!NI && j >= p.ni) break;
-        T e = eC[j] + J[j];
+        T e = (T)p.Dion[j] * lC[j] + (Rr[j] + J[j]);
         if ((DIM == 2 || z == p.nz - 1) && ly_sl<LY>(p, j)) {
             T s1 = (T)p.K1[j] + Cv[j];
             T s2 = (T)p.K2[j] + Cv[j];
