@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "simulated dyad-ms/s on 1 GPU (plain vs surrogate) at ≤2% Ca err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-COPY_GBS = 5265.0  # a plain 1 GiB device-to-device copy on MI355X (read + write), measured with rocprofv3 (kernel trace)
+COPY_GBS = 5314.0  # a plain 1 GiB device-to-device copy on MI355X (read + write), measured with rocprofv3 (kernel trace)
 
 
 def log(*a):
