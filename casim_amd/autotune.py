@@ -1,14 +1,19 @@
 """Autotuner of the state sweep's launch shape.
 
 For one batch shape it times every candidate (kernel form, threads per
-workgroup, x-planes per workgroup) with the profiled step entry point, which
-brackets each sweep launch with HIP events, and keeps the fastest.  Results
-are cached in tuning.json next to this file, keyed by grid, species layout,
-precision, site count and steps per pass; results of the step do not depend
-on the choice.
+workgroup, x-planes per workgroup) by the wall time of the fused step the
+solver runs (the captured graph, with split pipelining when it is on) and
+keeps the fastest; the sweep launch time from the profiled step entry point,
+which brackets each launch with HIP events, is recorded beside it.  The two
+can rank candidates differently: two half-batches on two streams overlap,
+so the shape with the shortest whole-batch launch is not always the one
+with the shortest step.  Results are cached in tuning.json next to this
+file, keyed by grid, species layout, precision, site count and steps per
+pass; results of the step do not depend on the choice.
 """
 import json
 import os
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
@@ -62,9 +67,12 @@ def apply(dev, spec):
 
 
 def tune(dev, spec, dt, steps=20, warmup=4, save=True, log=None):
-    """Time every candidate on `dev` (a DyadHIP after its first Update).
-    The prefetch depth belongs to the one-step sweep only, so with two steps
-    per pass each (planes, threads) shape is timed once."""
+    """Time every candidate on `dev` (a DyadHIP after its first Update):
+    `steps` profiled steps for the sweep's launch time, then the fastest of
+    three calls of 2 `steps` fused steps, after one that captures their
+    graph, for the step's wall time, which picks the shape.  The prefetch depth belongs to the
+    one-step sweep only, so with two steps per pass each (planes, threads)
+    shape is timed once."""
     res = []
     cands = candidates(spec)
     if dev.StepsPerPass() == 2:
@@ -77,10 +85,20 @@ def tune(dev, spec, dt, steps=20, warmup=4, save=True, log=None):
             continue
         dev.StepProfiled(dt, warmup)
         ms, _ = dev.StepProfiled(dt, steps)
-        res.append({"planes": planes, "threads": th, "prefetch": pf, "sweep_ms": ms / steps})
+        dev.Step(dt, 2 * steps)
+        dev.Synchronize()
+        walls = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            dev.Step(dt, 2 * steps)
+            dev.Synchronize()
+            walls.append(time.perf_counter() - t0)
+        step_ms = min(walls) * 1e3 / (2 * steps)
+        res.append({"planes": planes, "threads": th, "prefetch": pf, "sweep_ms": ms / steps, "step_ms": step_ms})
         if log:
-            log("tune planes=%d threads=%d prefetch=%d: %.4f ms" % (planes, th, pf, ms / steps))
-    best = min(res, key=lambda r: r["sweep_ms"])
+            log("tune planes=%d threads=%d prefetch=%d: sweep %.4f ms, step %.4f ms per step"
+                % (planes, th, pf, ms / steps, step_ms))
+    best = min(res, key=lambda r: r["step_ms"])
     dev.SetTuning(best["planes"], best["threads"], best["prefetch"])
     if save:
         c = load_cache()

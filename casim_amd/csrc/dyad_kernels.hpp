@@ -780,8 +780,13 @@ __global__ __launch_bounds__(512) void k_sweep(const T *__restrict__ gin, T *__r
 // the pool's value, which the channel call before the pass has advanced.  Chunks of one site recompute one halo plane of
 // step k+1 on each side.  Node arithmetic is node_update, as in k_sweep, so
 // two steps here equal two k_sweep steps bit for bit.
+// The FP64 3D form with one plane slot per thread holds 81 VGPRs, one over
+// what six waves per SIMD allow: the hint brings it to 80 without spills, so
+// a CU holds three 7-wave workgroups (448 threads, one site each) instead of
+// two (4096-site batch with two half-batch streams: 52.3 against 51.6
+// simulated dyad-ms/s).  Other forms keep the compiler's choice.
 #ifndef CASIM_SWEEP2_ATTR
-#define CASIM_SWEEP2_ATTR
+#define CASIM_SWEEP2_ATTR __attribute__((amdgpu_waves_per_eu(SL == 1 && DIM == 3 && sizeof(T) == 8 ? 6 : 1)))
 #endif
 template <typename T, int DIM, int NI, int NB, int NGT, int NFT, int LY, int SL, bool RING>
 __global__ __launch_bounds__(512) CASIM_SWEEP2_ATTR void k_sweep2(const T *__restrict__ gin, T *__restrict__ gout,
