@@ -107,44 +107,86 @@ __device__ double stationary_mouth(const QsDev &q, int off, int n, int i) {
     return c + q.st_bg;
 }
 
-// Features of every channel of every active site: lanes over channels, the
-// 16 nearest open channels kept by insertion (stable for equal distances,
-// like a stable argsort).
-__global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
-    __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
-    __shared__ int so[QS_MAXCH], ol[QS_MAXCH];
-    const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
-    const int base = q.crow[s];
-    if (base < 0) return;
-    const int off = q.off[s], n = q.off[s + 1] - off;
-    // the open channels in index order (a ballot per 64 channels), so that the
-    // sums below run over the open ones only, in the order of a full scan
-    int nopen = 0;
-    for (int c0 = 0; c0 < n; c0 += 64) {
-        const int i = c0 + (int)threadIdx.x;
-        const bool op = i < n && q.state[off + i] == 1;
-        if (i < n) {
-            sx[i] = q.x[off + i];
-            sy[i] = q.y[off + i];
-            so[i] = op;
-        }
-        const unsigned long long m = __ballot(op);
-        if (op) ol[nopen + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = i;
-        nopen += __popcll(m);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 64) {
-        float *f = q.X + (size_t)(base + i) * q.nf;
-        const float xi = sx[i], yi = sy[i];
-        float rk[QS_NEAR], dxk[QS_NEAR], dyk[QS_NEAR];
+// Nearest-open list of one channel: (r, dx, dy) of the QS_NEAR nearest open
+// channels in ascending r, kept by insertion; an entry goes after the equal
+// ones already held, so inserting in index order gives a stable argsort.
+struct NearList {
+    float rk[QS_NEAR], dxk[QS_NEAR], dyk[QS_NEAR];
+    __device__ __forceinline__ void clear() {
 #pragma unroll
         for (int k = 0; k < QS_NEAR; ++k) {
             rk[k] = INFINITY;
             dxk[k] = 0.f;
             dyk[k] = 0.f;
         }
-        float sinv = 0.f, sexp = 0.f, lin = 0.f;
-        for (int o = 0; o < nopen; ++o) {
+    }
+    __device__ __forceinline__ bool insert(float r, float dx, float dy) {
+        if (!(r < rk[QS_NEAR - 1])) return false;
+        rk[QS_NEAR - 1] = r;
+        dxk[QS_NEAR - 1] = dx;
+        dyk[QS_NEAR - 1] = dy;
+#pragma unroll
+        for (int k = QS_NEAR - 1; k > 0; --k) {
+            if (rk[k] < rk[k - 1]) {
+                float t0 = rk[k]; rk[k] = rk[k - 1]; rk[k - 1] = t0;
+                t0 = dxk[k]; dxk[k] = dxk[k - 1]; dxk[k - 1] = t0;
+                t0 = dyk[k]; dyk[k] = dyk[k - 1]; dyk[k - 1] = t0;
+            }
+        }
+        return true;
+    }
+};
+
+// Features of every channel of every active site, one workgroup of four
+// waves per site: lanes over channels (64-channel chunks), and when many
+// channels are open the open list is split into consecutive parts, one wave
+// each.  The wave of part 0 then adds the other parts' sums in part order
+// and inserts their nearest lists in part order, which gives the stable
+// nearest list of the whole scan (the sums regroup, to rounding).
+constexpr int QS_FWAVES = 4;
+constexpr int QS_PSTRIDE = 3 + 3 * QS_NEAR;  // sinv, sexp, lin, then r, dx, dy per entry
+__global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
+    __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
+    __shared__ int so[QS_MAXCH], ol[QS_MAXCH];
+    __shared__ int s_nopen;
+    __shared__ float part[QS_FWAVES - 1][64 * QS_PSTRIDE];
+    const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
+    const int base = q.crow[s];
+    if (base < 0) return;
+    const int off = q.off[s], n = q.off[s + 1] - off;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the open channels in index order (a ballot per 64 channels)
+    if (w == 0) {
+        int nopen = 0;
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int i = c0 + lane;
+            const bool op = i < n && q.state[off + i] == 1;
+            if (i < n) {
+                sx[i] = q.x[off + i];
+                sy[i] = q.y[off + i];
+                so[i] = op;
+            }
+            const unsigned long long m = __ballot(op);
+            if (op) ol[nopen + __popcll(m & ((1ull << lane) - 1ull))] = i;
+            nopen += __popcll(m);
+        }
+        if (lane == 0) s_nopen = nopen;
+    }
+    __syncthreads();
+    const int nopen = s_nopen;
+    const int nchunk = (n + 63) / 64;  // 1 or 2
+    const int parts = nopen >= 32 ? QS_FWAVES / nchunk : 1;
+    const int c = w / parts, h = w - c * parts;
+    const int i = c * 64 + lane;
+    const bool mine = c < nchunk && i < n;
+    const int len = (nopen + parts - 1) / parts;
+    const int o0 = h * len, o1 = min(nopen, o0 + len);
+    NearList nl;
+    nl.clear();
+    float sinv = 0.f, sexp = 0.f, lin = 0.f;
+    const float xi = mine ? sx[i] : 0.f, yi = mine ? sy[i] : 0.f;
+    if (mine) {
+        for (int o = o0; o < o1; ++o) {
             const int j = ol[o];
             if (j == i) continue;
             const float dx = sx[j] - xi, dy = sy[j] - yi;
@@ -153,40 +195,57 @@ __global__ __launch_bounds__(64) void k_qs_features(QsDev q) {
             sinv += 1.f / r;
             sexp += expf(-r);
             if (q.lin_n) lin += lin_added(q, r2);
-            if (r < rk[QS_NEAR - 1]) {
-                rk[QS_NEAR - 1] = r;
-                dxk[QS_NEAR - 1] = dx;
-                dyk[QS_NEAR - 1] = dy;
+            nl.insert(r, dx, dy);
+        }
+    }
+    if (parts > 1) {
+        if (h > 0 && mine) {
+            float *pp = part[c * (parts - 1) + h - 1] + lane * QS_PSTRIDE;
+            pp[0] = sinv;
+            pp[1] = sexp;
+            pp[2] = lin;
 #pragma unroll
-                for (int k = QS_NEAR - 1; k > 0; --k) {
-                    if (rk[k] < rk[k - 1]) {
-                        float t0 = rk[k]; rk[k] = rk[k - 1]; rk[k - 1] = t0;
-                        t0 = dxk[k]; dxk[k] = dxk[k - 1]; dxk[k - 1] = t0;
-                        t0 = dyk[k]; dyk[k] = dyk[k - 1]; dyk[k - 1] = t0;
-                    }
-                }
+            for (int k = 0; k < QS_NEAR; ++k) {
+                pp[3 + 3 * k] = nl.rk[k];
+                pp[4 + 3 * k] = nl.dxk[k];
+                pp[5 + 3 * k] = nl.dyk[k];
             }
         }
-        f[0] = (float)so[i];
-        f[1] = (float)(q.jsr[s] / 1000.0);
-        f[2] = nopen / 100.f;
-        f[3] = sinv;
-        f[4] = sexp;
-        f[5] = xi;
-        f[6] = yi;
-        f[7] = q.half - fmaxf(fabsf(xi), fabsf(yi));
+        __syncthreads();
+        if (h == 0 && mine) {
+            for (int hh = 1; hh < parts; ++hh) {
+                const float *pp = part[c * (parts - 1) + hh - 1] + lane * QS_PSTRIDE;
+                sinv += pp[0];
+                sexp += pp[1];
+                lin += pp[2];
+                // the part's list is ascending: the first entry that does not
+                // enter ends it
+                for (int k = 0; k < QS_NEAR; ++k)
+                    if (!nl.insert(pp[3 + 3 * k], pp[4 + 3 * k], pp[5 + 3 * k])) break;
+            }
+        }
+    }
+    if (h != 0 || !mine) return;
+    float *f = q.X + (size_t)(base + i) * q.nf;
+    f[0] = (float)so[i];
+    f[1] = (float)(q.jsr[s] / 1000.0);
+    f[2] = nopen / 100.f;
+    f[3] = sinv;
+    f[4] = sexp;
+    f[5] = xi;
+    f[6] = yi;
+    f[7] = q.half - fmaxf(fabsf(xi), fabsf(yi));
 #pragma unroll
-        for (int k = 0; k < QS_NEAR; ++k) {
-            const bool ok = rk[k] < INFINITY;
-            f[8 + 3 * k] = ok ? dxk[k] : 0.f;
-            f[9 + 3 * k] = ok ? dyk[k] : 0.f;
-            f[10 + 3 * k] = ok ? 1.f / rk[k] : 0.f;
-        }
-        if (q.lin_n) {
-            // ln of the superposition estimate: background + jSR-scaled sum over open RyRs (self included)
-            if (so[i]) lin += q.lin_tab[0];
-            f[QS_F] = logf(q.lin_bg + (float)(q.jsr[s] / (double)q.lin_jsr) * lin);
-        }
+    for (int k = 0; k < QS_NEAR; ++k) {
+        const bool ok = nl.rk[k] < INFINITY;
+        f[8 + 3 * k] = ok ? nl.dxk[k] : 0.f;
+        f[9 + 3 * k] = ok ? nl.dyk[k] : 0.f;
+        f[10 + 3 * k] = ok ? 1.f / nl.rk[k] : 0.f;
+    }
+    if (q.lin_n) {
+        // ln of the superposition estimate: background + jSR-scaled sum over open RyRs (self included)
+        if (so[i]) lin += q.lin_tab[0];
+        f[QS_F] = logf(q.lin_bg + (float)(q.jsr[s] / (double)q.lin_jsr) * lin);
     }
 }
 
@@ -664,7 +723,7 @@ int casim_qs_run(CasimQS *h, CasimMLP *m, int32_t max_rounds, int32_t wait_batch
         while (r < max_rounds && cnt[0] > 0 && cnt[1] < batch && cnt[1] < cnt[0]) {
             h->q.R = cnt[2];
             if (!stationary) {
-                hipLaunchKernelGGL(k_qs_features, dim3(cnt[3]), dim3(64), 0, h->stream, h->q);
+                hipLaunchKernelGGL(k_qs_features, dim3(cnt[3]), dim3(64 * QS_FWAVES), 0, h->stream, h->q);
                 HIPCK(hipGetLastError());
                 mlp_forward_device(m, h->q.X, cnt[2], h->q.out, h->stream);
             }
@@ -841,7 +900,7 @@ int casim_qs_get_features(CasimQS *h, float *out) {
         h->use();
         hipLaunchKernelGGL(k_qs_plan, dim3(1), dim3(1024), 0, h->stream, h->q, 1, h->d_count);
         HIPCK(hipGetLastError());
-        hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64), 0, h->stream, h->q);
+        hipLaunchKernelGGL(k_qs_features, dim3(h->q.S), dim3(64 * QS_FWAVES), 0, h->stream, h->q);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(out, h->q.X, sizeof(float) * (size_t)h->nch * h->q.nf, hipMemcpyDeviceToHost,
                              h->stream));
