@@ -186,21 +186,48 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ X, 
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-            const uint16_t *wl = wt + a.wtoff[L];
-            for (int k0 = 0; k0 < K; k0 += 32) {
+            // a lane's weight rows for the wave's four column blocks (blocks
+            // past nj repeat the last one; their products are not used)
+            const uint16_t *wrow[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                wrow[j] = wt + a.wtoff[L] + (size_t)(n0 + min(j, nj - 1) * 16 + (l & 15)) * K + (l >> 4) * 8;
+            // the next k-block's weights are in flight while this one's
+            // products run: two register sets used in turn (a copy from one
+            // to the other would wait for the loads), four loads per block,
+            // issued without branches (past the end they repeat the last block)
+            bf16x8_t bv[4], bn[4];
+            auto load_b = [&](bf16x8_t (&b)[4], int k) {
+                k = k < K ? k : K - 32;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j] + k);
+            };
+            auto products = [&](const bf16x8_t (&b)[4], int k) {
                 bf16x8_t av[2];
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
-                    av[i] = *reinterpret_cast<const bf16x8_t *>(&H[src][i * 16 + (l & 15)][k0 + (l >> 4) * 8]);
+                    av[i] = *reinterpret_cast<const bf16x8_t *>(&H[src][i * 16 + (l & 15)][k + (l >> 4) * 8]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (j >= nj) break;
-                    const bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(
-                        wl + (size_t)(n0 + j * 16 + (l & 15)) * K + k0 + (l >> 4) * 8);
+                for (int j = 0; j < 4; ++j)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[i][j], 0, 0, 0);
-                }
+                    for (int i = 0; i < 2; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], b[j], acc[i][j], 0, 0, 0);
+            };
+            // (scheduling barriers keep each block's loads ahead of the other
+            // block's products)
+            load_b(bv, 0);
+            int k0 = 0;
+            for (; k0 + 64 <= K; k0 += 64) {
+                load_b(bn, k0 + 32);
+                __builtin_amdgcn_sched_barrier(0);
+                products(bv, k0);
+                __builtin_amdgcn_sched_barrier(0);
+                load_b(bv, k0 + 64);
+                __builtin_amdgcn_sched_barrier(0);
+                products(bn, k0 + 32);
+                __builtin_amdgcn_sched_barrier(0);
             }
+            if (k0 < K) products(bv, k0);  // an odd number of 32-wide blocks
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (j >= nj) break;
