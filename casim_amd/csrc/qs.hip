@@ -467,9 +467,12 @@ __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restr
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
         const int v = t >= d ? part[t - d] : 0, vn = t >= d ? pn[t - d] : 0;
+        const int va = t >= d ? pa[t - d] : 0, vw = t >= d ? pw[t - d] : 0;
         __syncthreads();
         part[t] += v;
         pn[t] += vn;
+        pa[t] += va;
+        pw[t] += vw;
         __syncthreads();
     }
     int r = part[t] - rows, c = pn[t] - nc;
@@ -484,13 +487,8 @@ __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restr
         }
     }
     if (t == 0) {
-        int ta = 0, tw = 0;
-        for (int k = 0; k < 1024; ++k) {
-            ta += pa[k];
-            tw += pw[k];
-        }
-        cnt[0] = ta;
-        cnt[1] = tw;
+        cnt[0] = pa[1023];
+        cnt[1] = pw[1023];
         cnt[2] = part[1023];
         cnt[3] = pn[1023];  // sites with rows: the grid of k_qs_features and k_qs_event
     }
