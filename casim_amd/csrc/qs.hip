@@ -272,8 +272,9 @@ __device__ double jsr_advance(const QsDev &q, double c, double geff, double dt) 
 // for the full solver when the rate-weighted mean of the relative spreads,
 // sum_i (rate_i / total) spread_i, exceeds the threshold.
 __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
-    __shared__ double rate[QS_MAXCH];
+    __shared__ double rate[QS_MAXCH], wv[QS_MAXCH];
     __shared__ float spv[QS_MAXCH], cav[QS_MAXCH];
+    __shared__ double s_total;
     const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
     const int lane = threadIdx.x;
     const int base = q.crow[s];
@@ -316,9 +317,19 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
         rate[i] = channel_rate(q.model, q.par, st, (double)ca, q.mg);
     }
     __syncthreads();
+    // the total in channel order on one lane, then every channel's share of it
+    // on all lanes (the serial sums below then only add)
+    if (lane == 0) {
+        double tot = 0.0;
+        for (int i = 0; i < n; ++i) tot += rate[i];
+        s_total = tot;
+    }
+    __syncthreads();
+    const double total = s_total;
+    if (total > 0.0)
+        for (int i = lane; i < n; i += 64) wv[i] = rate[i] / total;
+    __syncthreads();
     if (lane != 0) return;
-    double total = 0.0;
-    for (int i = 0; i < n; ++i) total += rate[i];
     double *st = q.stats + (size_t)s * QS_NSTAT;
     if (!(total > 0.0)) {
         // nothing can happen any more: N stays until t_max, and the open
@@ -339,7 +350,7 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     }
     if (wait == 0) {
         double m = 0.0;
-        for (int i = 0; i < n; ++i) m += rate[i] / total * (double)spv[i];
+        for (int i = 0; i < n; ++i) m += wv[i] * (double)spv[i];
         if (m > (double)q.thresh) {
             q.waiting[s] = 1;
             return;
@@ -356,7 +367,7 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     double pc = 0.0;
     int idx = 0;
     for (int i = 0; i < n; ++i) {
-        const double w = rate[i] / total;
+        const double w = wv[i];
         if (pc <= r && r <= pc + w) {
             idx = i;
             break;
