@@ -143,8 +143,11 @@ __global__ void k_bias_act(float *__restrict__ Z, const float *__restrict__ b, u
 // zero-padded ([n][k], k_wt) so a lane's eight k-values of an MFMA operand are
 // one 16-byte load.  Per output element the operations and their order are
 // those of k_gather + k_gemm + k_bias_act, so the outputs are the same bits.
-// Grid: (row tiles, members); four waves, wave w computes columns 64w .. +63.
-constexpr int IMR = 32, IMAXW = 256, IPADK = 8, IMAXM = 8, IMAXL = 8;
+// Grid: (row tiles, members); sixteen waves, wave w computes the 16 columns
+// 16w .. +15 of a layer, and a lane loads all of the layer's k-blocks of its
+// weight row at once (at most IMAXW / 32 = 8), so a layer waits for the
+// weights once: with few rows a workgroup is a latency chain of layers.
+constexpr int IMR = 32, IMAXW = 256, IPADK = 8, IMAXM = 8, IMAXL = 8, IWAVES = IMAXW / 16;
 struct InferArgs {
     const uint16_t *wt[IMAXM];  // member's transposed weights
     const float *p[IMAXM];      // member's FP32 parameters (biases at boff)
@@ -160,13 +163,13 @@ __global__ void k_wt(const uint16_t *__restrict__ W, uint16_t *__restrict__ WT, 
     WT[i] = (n < fout && k < fin) ? W[(size_t)k * fout + n] : (uint16_t)0;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ X, int rows, float *__restrict__ out,
-                                                   int ostride, InferArgs a) {
+__global__ __launch_bounds__(64 * IWAVES) void k_mlp_infer(const float *__restrict__ X, int rows,
+                                                            float *__restrict__ out, int ostride, InferArgs a) {
     __shared__ __attribute__((aligned(16))) uint16_t H[2][IMR][IMAXW + IPADK];
     const int m0 = blockIdx.x * IMR, e = blockIdx.y;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int kp0 = a.kp[0];
-    for (int i = t; i < IMR * kp0; i += 256) {
+    for (int i = t; i < IMR * kp0; i += 64 * IWAVES) {
         const int r = i / kp0, f = i - r * kp0;
         uint16_t v = 0;
         if (m0 + r < rows && f < a.n_in) v = f2bf((X[(size_t)(m0 + r) * a.n_in + f] - a.mean[f]) * a.istd[f]);
@@ -178,68 +181,40 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ X, 
     for (int L = 0; L < a.nl; ++L) {
         const int src = L & 1, K = a.kp[L], N = a.fout[L], NP = (N + 15) & ~15;
         const bool last = L == a.nl - 1;
-        const int n0 = w * 64;
+        const int n0 = w * 16;
         if (n0 < NP) {
-            const int nj = min(4, (NP - n0) >> 4);
-            f32x4_t acc[2][4];
+            const int nk = K >> 5;  // 32-wide k-blocks, at most 8
+            const uint16_t *wrow = wt + a.wtoff[L] + (size_t)(n0 + (l & 15)) * K + (l >> 4) * 8;
+            // every k-block of the lane's weight row in flight together (blocks
+            // past nk repeat the last one and are not used)
+            bf16x8_t bw[IMAXW / 32];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int kb = 0; kb < IMAXW / 32; ++kb)
+                bw[kb] = *reinterpret_cast<const bf16x8_t *>(wrow + 32 * (kb < nk ? kb : nk - 1));
+            // an empty use of every block keeps the loads here, together (the
+            // compiler would otherwise move each one next to its product)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-            // a lane's weight rows for the wave's four column blocks (blocks
-            // past nj repeat the last one; their products are not used)
-            const uint16_t *wrow[4];
+            for (int kb = 0; kb < IMAXW / 32; ++kb) asm volatile("" : : "v"(bw[kb]));
+            const float bias = pe[a.boff[L] + min(n0 + (l & 15), N - 1)];
+            f32x4_t acc[2] = {f32x4_t{0.f, 0.f, 0.f, 0.f}, f32x4_t{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                wrow[j] = wt + a.wtoff[L] + (size_t)(n0 + min(j, nj - 1) * 16 + (l & 15)) * K + (l >> 4) * 8;
-            // the next k-block's weights are in flight while this one's
-            // products run: two register sets used in turn (a copy from one
-            // to the other would wait for the loads), four loads per block,
-            // issued without branches (past the end they repeat the last block)
-            bf16x8_t bv[4], bn[4];
-            auto load_b = [&](bf16x8_t (&b)[4], int k) {
-                k = k < K ? k : K - 32;
+            for (int kb = 0; kb < IMAXW / 32; ++kb) {
+                if (kb >= nk) break;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j] + k);
-            };
-            auto products = [&](const bf16x8_t (&b)[4], int k) {
-                bf16x8_t av[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-                    av[i] = *reinterpret_cast<const bf16x8_t *>(&H[src][i * 16 + (l & 15)][k + (l >> 4) * 8]);
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], b[j], acc[i][j], 0, 0, 0);
-            };
-            // (scheduling barriers keep each block's loads ahead of the other
-            // block's products)
-            load_b(bv, 0);
-            int k0 = 0;
-            for (; k0 + 64 <= K; k0 += 64) {
-                load_b(bn, k0 + 32);
-                __builtin_amdgcn_sched_barrier(0);
-                products(bv, k0);
-                __builtin_amdgcn_sched_barrier(0);
-                load_b(bv, k0 + 64);
-                __builtin_amdgcn_sched_barrier(0);
-                products(bn, k0 + 32);
-                __builtin_amdgcn_sched_barrier(0);
+                for (int i = 0; i < 2; ++i) {
+                    const bf16x8_t av =
+                        *reinterpret_cast<const bf16x8_t *>(&H[src][i * 16 + (l & 15)][32 * kb + (l >> 4) * 8]);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[kb], acc[i], 0, 0, 0);
+                }
             }
-            if (k0 < K) products(bv, k0);  // an odd number of 32-wide blocks
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j >= nj) break;
-                const int col = n0 + j * 16 + (l & 15);
-                if (col >= N) continue;
-                const float bias = pe[a.boff[L] + col];
+            const int col = n0 + (l & 15);
+            if (col < N) {
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = i * 16 + (l >> 4) * 4 + r;
-                        const float z = acc[i][j][r] + bias;
+                        const float z = acc[i][r] + bias;
                         if (last) {
                             if (m0 + row < rows) out[(size_t)e * ostride + m0 + row] = z;
                         } else {
@@ -415,7 +390,7 @@ void CasimMLP::infer(const float *x, int B, float *out, int ostride) {
     a.n_in = n_in;
     a.mean = mean;
     a.istd = istd;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((B + IMR - 1) / IMR), (unsigned)members), dim3(256), 0, stream, x,
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((B + IMR - 1) / IMR), (unsigned)members), dim3(64 * IWAVES), 0, stream, x,
                        B, out, ostride, a);
     HIPCK(hipGetLastError());
 }
