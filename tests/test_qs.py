@@ -198,6 +198,32 @@ def test_device_features_match_numpy(gpu):
 
 
 @pytest.mark.gpu
+def test_device_features_with_many_open_channels_match_numpy(gpu):
+    """Sites of 40..100 RyRs (up to two 64-channel chunks) with up to all of
+    them open, so that the open list is split over the workgroup's waves and
+    merged; on the lattice equal distances are common, so the nearest list's
+    stable order is checked too.  Sums over up to 100 open RyRs regroup, to
+    FP32 rounding."""
+    from casim_amd import surrogate
+    spec = workloads.batched_dyads(n_sites=16, geometry_seed=11, n_min=40, n_max=100)
+    jsr = np.linspace(300.0, 1100.0, spec.n_sites)
+    net = _net(zero_bias=8.0)  # mouth [Ca2+] far above the opening threshold: events open RyRs
+    qs = surrogate.QuasiStationarySites(spec, jsr, 50.0, net, spread_threshold=10.0)
+    most = 0
+    for _ in range(12):
+        qs.Round(8)
+        st = qs.GetChannelsStates()
+        dev = qs.GetFeatures()
+        ref = surrogate.batch_features(spec, st, jsr)
+        assert dev.shape == ref.shape
+        np.testing.assert_allclose(dev, ref, rtol=1e-5, atol=1e-5)
+        most = max(most, int((st == 1).sum(axis=1).max()))
+    assert most >= 32, most
+    assert max(len(s) for s in spec.sites) > 64
+    qs.close()
+
+
+@pytest.mark.gpu
 def test_device_superposition_feature_matches_numpy(gpu):
     from casim_amd import mlp, surrogate
     spec = _spec(32)
