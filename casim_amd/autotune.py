@@ -35,7 +35,7 @@ def candidates(spec):
             continue
         if slots > 1 and th * slots - P >= th // 2:
             continue  # more than half a block idle on the last slot
-        chunks = {0, (inner + 1) // 2, (inner + 3) // 4}
+        chunks = {0, (inner + 1) // 2, (inner + 2) // 3, (inner + 3) // 4}
         if spec.n_sites < 2048:  # few sites: shorter chunks give the 256 CUs more workgroups
             chunks |= {(inner + 7) // 8, (inner + 15) // 16}
         for chunk in sorted(chunks):
