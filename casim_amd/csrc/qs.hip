@@ -107,33 +107,31 @@ __device__ double stationary_mouth(const QsDev &q, int off, int n, int i) {
     return c + q.st_bg;
 }
 
-// Nearest-open list of one channel: (r, dx, dy) of the QS_NEAR nearest open
-// channels in ascending r, kept by insertion; an entry goes after the equal
-// ones already held, so inserting in index order gives a stable argsort.
+// Nearest-open list of one channel: the QS_NEAR nearest open channels as
+// keys (bits of r << 32 | channel index), ascending.  r >= 0, whose bits
+// order like its values, so the key order is r's with ties broken by the
+// channel index: the order of a stable argsort over the channels in index
+// order.  Kept by insertion; one compare and two selects per step.
 struct NearList {
-    float rk[QS_NEAR], dxk[QS_NEAR], dyk[QS_NEAR];
+    unsigned long long key[QS_NEAR];
     __device__ __forceinline__ void clear() {
 #pragma unroll
-        for (int k = 0; k < QS_NEAR; ++k) {
-            rk[k] = INFINITY;
-            dxk[k] = 0.f;
-            dyk[k] = 0.f;
-        }
+        for (int k = 0; k < QS_NEAR; ++k) key[k] = ~0ull;
     }
-    __device__ __forceinline__ bool insert(float r, float dx, float dy) {
-        if (!(r < rk[QS_NEAR - 1])) return false;
-        rk[QS_NEAR - 1] = r;
-        dxk[QS_NEAR - 1] = dx;
-        dyk[QS_NEAR - 1] = dy;
+    __device__ __forceinline__ bool insert(unsigned long long kk) {
+        if (!(kk < key[QS_NEAR - 1])) return false;
+        key[QS_NEAR - 1] = kk;
 #pragma unroll
         for (int k = QS_NEAR - 1; k > 0; --k) {
-            if (rk[k] < rk[k - 1]) {
-                float t0 = rk[k]; rk[k] = rk[k - 1]; rk[k - 1] = t0;
-                t0 = dxk[k]; dxk[k] = dxk[k - 1]; dxk[k - 1] = t0;
-                t0 = dyk[k]; dyk[k] = dyk[k - 1]; dyk[k - 1] = t0;
-            }
+            const unsigned long long lo = key[k] < key[k - 1] ? key[k] : key[k - 1];
+            const unsigned long long hi = key[k] < key[k - 1] ? key[k - 1] : key[k];
+            key[k - 1] = lo;
+            key[k] = hi;
         }
         return true;
+    }
+    static __device__ __forceinline__ unsigned long long make(float r, int j) {
+        return ((unsigned long long)__float_as_uint(r) << 32) | (unsigned)j;
     }
 };
 
@@ -144,12 +142,12 @@ struct NearList {
 // and inserts their nearest lists in part order, which gives the stable
 // nearest list of the whole scan (the sums regroup, to rounding).
 constexpr int QS_FWAVES = 4;
-constexpr int QS_PSTRIDE = 3 + 3 * QS_NEAR;  // sinv, sexp, lin, then r, dx, dy per entry
+constexpr int QS_PSTRIDE = 4 + 2 * QS_NEAR;  // sinv, sexp, lin, a pad word, then the keys (8-byte aligned)
 __global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
     __shared__ float sx[QS_MAXCH], sy[QS_MAXCH];
     __shared__ int so[QS_MAXCH], ol[QS_MAXCH];
     __shared__ int s_nopen;
-    __shared__ float part[QS_FWAVES - 1][64 * QS_PSTRIDE];
+    __shared__ __attribute__((aligned(16))) float part[QS_FWAVES - 1][64 * QS_PSTRIDE];
     const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
     const int base = q.crow[s];
     if (base < 0) return;
@@ -195,7 +193,7 @@ __global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
             sinv += 1.f / r;
             sexp += expf(-r);
             if (q.lin_n) lin += lin_added(q, r2);
-            nl.insert(r, dx, dy);
+            nl.insert(NearList::make(r, j));
         }
     }
     if (parts > 1) {
@@ -204,12 +202,9 @@ __global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
             pp[0] = sinv;
             pp[1] = sexp;
             pp[2] = lin;
+            unsigned long long *pk = reinterpret_cast<unsigned long long *>(pp + 4);
 #pragma unroll
-            for (int k = 0; k < QS_NEAR; ++k) {
-                pp[3 + 3 * k] = nl.rk[k];
-                pp[4 + 3 * k] = nl.dxk[k];
-                pp[5 + 3 * k] = nl.dyk[k];
-            }
+            for (int k = 0; k < QS_NEAR; ++k) pk[k] = nl.key[k];
         }
         __syncthreads();
         if (h == 0 && mine) {
@@ -220,8 +215,9 @@ __global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
                 lin += pp[2];
                 // the part's list is ascending: the first entry that does not
                 // enter ends it
+                const unsigned long long *pk = reinterpret_cast<const unsigned long long *>(pp + 4);
                 for (int k = 0; k < QS_NEAR; ++k)
-                    if (!nl.insert(pp[3 + 3 * k], pp[4 + 3 * k], pp[5 + 3 * k])) break;
+                    if (!nl.insert(pk[k])) break;
             }
         }
     }
@@ -237,10 +233,12 @@ __global__ __launch_bounds__(64 * QS_FWAVES) void k_qs_features(QsDev q) {
     f[7] = q.half - fmaxf(fabsf(xi), fabsf(yi));
 #pragma unroll
     for (int k = 0; k < QS_NEAR; ++k) {
-        const bool ok = nl.rk[k] < INFINITY;
-        f[8 + 3 * k] = ok ? nl.dxk[k] : 0.f;
-        f[9 + 3 * k] = ok ? nl.dyk[k] : 0.f;
-        f[10 + 3 * k] = ok ? 1.f / nl.rk[k] : 0.f;
+        const bool ok = nl.key[k] != ~0ull;
+        const int j = ok ? (int)(unsigned)nl.key[k] : i;
+        const float r = __uint_as_float((unsigned)(nl.key[k] >> 32));
+        f[8 + 3 * k] = ok ? sx[j] - xi : 0.f;
+        f[9 + 3 * k] = ok ? sy[j] - yi : 0.f;
+        f[10 + 3 * k] = ok ? 1.f / r : 0.f;
     }
     if (q.lin_n) {
         // ln of the superposition estimate: background + jSR-scaled sum over open RyRs (self included)
