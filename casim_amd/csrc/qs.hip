@@ -270,7 +270,7 @@ __device__ double jsr_advance(const QsDev &q, double c, double geff, double dt) 
 // for the full solver when the rate-weighted mean of the relative spreads,
 // sum_i (rate_i / total) spread_i, exceeds the threshold.
 __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
-    __shared__ double rate[QS_MAXCH], wv[QS_MAXCH];
+    __shared__ double rate[QS_MAXCH], wv[QS_MAXCH], pcv[QS_MAXCH];
     __shared__ float spv[QS_MAXCH], cav[QS_MAXCH];
     __shared__ double s_total;
     const int s = q.list[blockIdx.x];  // one workgroup per site with rows this round
@@ -327,51 +327,65 @@ __global__ __launch_bounds__(64) void k_qs_event(QsDev q) {
     if (total > 0.0)
         for (int i = lane; i < n; i += 64) wv[i] = rate[i] / total;
     __syncthreads();
-    if (lane != 0) return;
     double *st = q.stats + (size_t)s * QS_NSTAT;
-    if (!(total > 0.0)) {
-        // nothing can happen any more: N stays until t_max, and the open
-        // channels keep drawing on the jSR until then
-        st[QS_WQ] += st[QS_NCUR] * (q.t_max - fmin(st[QS_TCUR], q.t_max));
-        if (q.deplete && q.t[s] < q.t_max) {
-            const double c0 = q.jsr[s];
-            double geff = 0.0;
-            for (int i = 0; i < n; ++i)
-                if (q.state[off + i] == 1) geff += q.gch * (1.0 - (double)cav[i] / c0);
-            const double cw = jsr_advance(q, c0, geff, q.t_max - q.t[s]);
-            q.jsr[s] = cw;
-            if (cw < q.jsr_min[s]) q.jsr_min[s] = cw;
-        }
-        q.active[s] = 0;
-        q.waiting[s] = 0;
-        return;
-    }
-    if (wait == 0) {
-        double m = 0.0;
-        for (int i = 0; i < n; ++i) m += wv[i] * (double)spv[i];
-        if (m > (double)q.thresh) {
-            q.waiting[s] = 1;
-            return;
-        }
-    }
-    q.waiting[s] = 0;
-    // Gillespie step (QuasyStationaryReleaseSite.cpp:101-128)
+    // lane 0: the shares' running sums in channel order (pcv[i], the sum of
+    // the shares before channel i), the spread test and the event's draws;
+    // then every lane tests its channels against the draw and a ballot finds
+    // the first one, the channel the serial scan would stop at
+    int go = 0;
+    double u1 = 0.0, r = 0.0;
     uint32_t S6[6];
-    for (int k = 0; k < 6; ++k) S6[k] = q.rng[s * 6 + k];
-    const double u1 = mrg::draw(S6);
+    if (lane == 0) {
+        if (!(total > 0.0)) {
+            // nothing can happen any more: N stays until t_max, and the open
+            // channels keep drawing on the jSR until then
+            st[QS_WQ] += st[QS_NCUR] * (q.t_max - fmin(st[QS_TCUR], q.t_max));
+            if (q.deplete && q.t[s] < q.t_max) {
+                const double c0 = q.jsr[s];
+                double geff = 0.0;
+                for (int i = 0; i < n; ++i)
+                    if (q.state[off + i] == 1) geff += q.gch * (1.0 - (double)cav[i] / c0);
+                const double cw = jsr_advance(q, c0, geff, q.t_max - q.t[s]);
+                q.jsr[s] = cw;
+                if (cw < q.jsr_min[s]) q.jsr_min[s] = cw;
+            }
+            q.active[s] = 0;
+            q.waiting[s] = 0;
+        } else {
+            double pc = 0.0, m = 0.0;
+            for (int i = 0; i < n; ++i) {
+                pcv[i] = pc;
+                m += wv[i] * (double)spv[i];
+                pc += wv[i];
+            }
+            if (wait == 0 && m > (double)q.thresh) {
+                q.waiting[s] = 1;
+            } else {
+                q.waiting[s] = 0;
+                // Gillespie step (QuasyStationaryReleaseSite.cpp:101-128)
+                for (int k = 0; k < 6; ++k) S6[k] = q.rng[s * 6 + k];
+                u1 = mrg::draw(S6);
+                r = mrg::draw(S6);
+                go = 1;
+            }
+        }
+    }
+    go = __shfl(go, 0);
+    if (!go) return;  // the whole wave
+    r = __shfl(r, 0);
+    __syncthreads();
+    int idx = 0;
+    {
+        const int i0 = lane, i1 = 64 + lane;
+        const bool h0 = i0 < n && pcv[i0] <= r && r <= pcv[i0] + wv[i0];
+        const bool h1 = i1 < n && pcv[i1] <= r && r <= pcv[i1] + wv[i1];
+        const unsigned long long b0 = __ballot(h0), b1 = __ballot(h1);
+        if (b0) idx = __ffsll((long long)b0) - 1;
+        else if (b1) idx = 64 + __ffsll((long long)b1) - 1;
+    }
+    if (lane != 0) return;
     const double dtw = -log(u1) * (1.0 / total);
     const double t = q.t[s] + dtw;
-    const double r = mrg::draw(S6);
-    double pc = 0.0;
-    int idx = 0;
-    for (int i = 0; i < n; ++i) {
-        const double w = wv[i];
-        if (pc <= r && r <= pc + w) {
-            idx = i;
-            break;
-        }
-        pc += w;
-    }
     (void)mrg::draw(S6);  // MakeTransition(runif())
     if (q.deplete) {
         // the jSR over the interval before the event, with this interval's open set
