@@ -145,8 +145,8 @@ __global__ void k_bias_act(float *__restrict__ Z, const float *__restrict__ b, u
 // those of k_gather + k_gemm + k_bias_act, so the outputs are the same bits.
 // Grid: (row tiles, members); sixteen waves, wave w computes the 16 columns
 // 16w .. +15 of a layer, and a lane loads all of the layer's k-blocks of its
-// weight row at once (at most IMAXW / 32 = 8), so a layer waits for the
-// weights once: with few rows a workgroup is a latency chain of layers.
+// weight row at once (at most IMAXW / 32 = 8) while the layer before runs:
+// with few rows a workgroup is a latency chain of layers.
 constexpr int IMR = 32, IMAXW = 256, IPADK = 8, IMAXM = 8, IMAXL = 8, IWAVES = IMAXW / 16;
 struct InferArgs {
     const uint16_t *wt[IMAXM];  // member's transposed weights
@@ -168,6 +168,23 @@ __global__ __launch_bounds__(64 * IWAVES) void k_mlp_infer(const float *__restri
     __shared__ __attribute__((aligned(16))) uint16_t H[2][IMR][IMAXW + IPADK];
     const int m0 = blockIdx.x * IMR, e = blockIdx.y;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const uint16_t *wt = a.wt[e];
+    const float *pe = a.p[e];
+    // every k-block of the lane's weight row for layer L, and its bias, issued
+    // together (blocks past the layer's K repeat its last one, columns past
+    // its width the last row; neither is used)
+    auto fetch = [&](int L, bf16x8_t (&b)[IMAXW / 32], float &bias) {
+        const int K = a.kp[L], N = a.fout[L], NP = (N + 15) & ~15, nk = K >> 5;
+        const int col = min(w * 16 + (l & 15), NP - 1);
+        const uint16_t *wrow = wt + a.wtoff[L] + (size_t)col * K + (l >> 4) * 8;
+#pragma unroll
+        for (int kb = 0; kb < IMAXW / 32; ++kb)
+            b[kb] = *reinterpret_cast<const bf16x8_t *>(wrow + 32 * (kb < nk ? kb : nk - 1));
+        bias = pe[a.boff[L] + min(col, N - 1)];
+    };
+    bf16x8_t bw[IMAXW / 32], bn[IMAXW / 32];
+    float bias, bias_n;
+    fetch(0, bw, bias);  // in flight while the input is staged
     const int kp0 = a.kp[0];
     for (int i = t; i < IMR * kp0; i += 64 * IWAVES) {
         const int r = i / kp0, f = i - r * kp0;
@@ -176,26 +193,16 @@ __global__ __launch_bounds__(64 * IWAVES) void k_mlp_infer(const float *__restri
         H[0][r][f] = v;
     }
     __syncthreads();
-    const uint16_t *wt = a.wt[e];
-    const float *pe = a.p[e];
     for (int L = 0; L < a.nl; ++L) {
         const int src = L & 1, K = a.kp[L], N = a.fout[L], NP = (N + 15) & ~15;
         const bool last = L == a.nl - 1;
         const int n0 = w * 16;
+        // the next layer's weights are in flight during this layer (the last
+        // layer fetches itself again, so that the count of loads in flight is
+        // the same every layer)
+        fetch(last ? L : L + 1, bn, bias_n);
         if (n0 < NP) {
             const int nk = K >> 5;  // 32-wide k-blocks, at most 8
-            const uint16_t *wrow = wt + a.wtoff[L] + (size_t)(n0 + (l & 15)) * K + (l >> 4) * 8;
-            // every k-block of the lane's weight row in flight together (blocks
-            // past nk repeat the last one and are not used)
-            bf16x8_t bw[IMAXW / 32];
-#pragma unroll
-            for (int kb = 0; kb < IMAXW / 32; ++kb)
-                bw[kb] = *reinterpret_cast<const bf16x8_t *>(wrow + 32 * (kb < nk ? kb : nk - 1));
-            // an empty use of every block keeps the loads here, together (the
-            // compiler would otherwise move each one next to its product)
-#pragma unroll
-            for (int kb = 0; kb < IMAXW / 32; ++kb) asm volatile("" : : "v"(bw[kb]));
-            const float bias = pe[a.boff[L] + min(n0 + (l & 15), N - 1)];
             f32x4_t acc[2] = {f32x4_t{0.f, 0.f, 0.f, 0.f}, f32x4_t{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
             for (int kb = 0; kb < IMAXW / 32; ++kb) {
@@ -224,6 +231,9 @@ __global__ __launch_bounds__(64 * IWAVES) void k_mlp_infer(const float *__restri
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < IMAXW / 32; ++kb) bw[kb] = bn[kb];
+        bias = bias_n;
     }
 }
 
