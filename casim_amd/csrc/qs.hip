@@ -468,9 +468,9 @@ __global__ void k_qs_init(QsDev q, const int *__restrict__ init, const uint32_t 
 // not wait for the full solver, packed in site order (one block).  cnt[0..2]:
 // active sites, waiting sites, rows.
 __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restrict__ cnt) {
-    __shared__ int part[1024], pn[1024];
-    __shared__ int pa[1024], pw[1024];
-    const int t = threadIdx.x;
+    // per-wave totals of the four counts, then their running sums (16 waves)
+    __shared__ int wsum[4][16], wpre[4][16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int per = (q.S + 1023) / 1024;
     const int s0 = t * per, s1 = min(q.S, s0 + per);
     int rows = 0, na = 0, nw = 0, nc = 0;
@@ -483,22 +483,28 @@ __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restr
             ++nc;
         }
     }
-    part[t] = rows;
-    pn[t] = nc;
-    pa[t] = na;
-    pw[t] = nw;
+    // inclusive scans across the wave (thread order = site order)
+    int v[4] = {rows, nc, na, nw};
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = __shfl_up(v[k], d);
+            if (lane >= d) v[k] += u;
+        }
+    if (lane == 63)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wsum[k][wv] = v[k];
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? part[t - d] : 0, vn = t >= d ? pn[t - d] : 0;
-        const int va = t >= d ? pa[t - d] : 0, vw = t >= d ? pw[t - d] : 0;
-        __syncthreads();
-        part[t] += v;
-        pn[t] += vn;
-        pa[t] += va;
-        pw[t] += vw;
-        __syncthreads();
+    if (t < 4) {
+        int acc = 0;
+        for (int j = 0; j < 16; ++j) {
+            wpre[t][j] = acc;
+            acc += wsum[t][j];
+        }
     }
-    int r = part[t] - rows, c = pn[t] - nc;
+    __syncthreads();
+    int r = wpre[0][wv] + v[0] - rows, c = wpre[1][wv] + v[1] - nc;
     for (int s = s0; s < s1; ++s) {
         const bool act = q.active[s] != 0, w = act && q.waiting[s] == 1;
         if (all || (act && !w)) {
@@ -509,11 +515,11 @@ __global__ __launch_bounds__(1024) void k_qs_plan(QsDev q, int all, int *__restr
             q.crow[s] = -1;
         }
     }
-    if (t == 0) {
-        cnt[0] = pa[1023];
-        cnt[1] = pw[1023];
-        cnt[2] = part[1023];
-        cnt[3] = pn[1023];  // sites with rows: the grid of k_qs_features and k_qs_event
+    if (t == 1023) {
+        cnt[0] = wpre[2][15] + v[2];
+        cnt[1] = wpre[3][15] + v[3];
+        cnt[2] = wpre[0][15] + v[0];
+        cnt[3] = wpre[1][15] + v[1];  // sites with rows: the grid of k_qs_features and k_qs_event
     }
 }
 
