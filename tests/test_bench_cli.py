@@ -32,3 +32,47 @@ def test_committed_tuning_covers_the_benchmark_shapes():
         t = autotune.lookup(spec)
         assert t is not None, autotune.key(spec)
         assert t["threads"] % 64 == 0 and t["planes"] >= 0 and t.get("prefetch", 1) >= 1
+
+
+class _TimedDevice:
+    """Stand-in for DyadHIP in the autotuner: each launch shape has a sweep
+    launch time and a fused step time, and the step calls sleep for it."""
+
+    def __init__(self, sweep_ms, step_ms, per_pass=2):
+        self.sweep_ms, self.step_ms, self.per_pass = sweep_ms, step_ms, per_pass
+        self.shape, self.calls = None, []
+
+    def StepsPerPass(self):
+        return self.per_pass
+
+    def SetTuning(self, planes, threads, prefetch):
+        self.shape = (planes, threads)
+
+    def StepProfiled(self, dt, k):
+        return self.sweep_ms.get(self.shape, 1.0) * k, 0.0
+
+    def Step(self, dt, k):
+        import time
+        self.calls.append((self.shape, k))
+        time.sleep(self.step_ms.get(self.shape, 1.0) * k * 1e-3)
+
+    def Synchronize(self):
+        pass
+
+
+def test_autotuner_picks_by_the_fused_step_and_records_both_times():
+    """The pick is the shape with the shortest fused step even where another
+    has the shorter sweep launch; each candidate's graph is captured by one
+    call before three timed ones."""
+    from casim_amd import autotune, workloads
+    spec = workloads.coupled_dyads(512)
+    cands = autotune.candidates(spec)
+    assert (67, 256, 1) in cands and (100, 256, 1) in cands  # thirds and halves of the 199 planes
+    sweep = {(100, 256): 0.10, (67, 256): 0.15}
+    step = {(100, 256): 0.30, (67, 256): 0.20}
+    dev = _TimedDevice(sweep, step)
+    best, res = autotune.tune(dev, spec, 1e-6, steps=5, warmup=1, save=False)
+    assert (best["planes"], best["threads"]) == (67, 256)
+    r100 = next(r for r in res if (r["planes"], r["threads"]) == (100, 256))
+    assert r100["sweep_ms"] < best["sweep_ms"] and r100["step_ms"] > best["step_ms"]
+    assert [k for s, k in dev.calls if s == (67, 256)] == [10, 10, 10, 10]
