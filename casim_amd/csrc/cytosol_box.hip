@@ -340,11 +340,14 @@ void box_step(CasimDyad *d, double dt) {
     const BoxArgs &a = b->a;
     const bool small = a.ni == 1 && a.nbb <= 3;
     const int nt = small ? 4 : MAXI + MAXB;  // fields per LDS node
-    // y-tiles of ty rows (+ 2 halo rows): about 512 nodes, and two LDS planes
-    // within 64 KB; nx <= 64 (attach)
-    const int rows = std::min(512 / a.nx, (int)(65536 / (2 * sizeof(double) * nt * a.nx)));
-    const int ty = std::min(a.ny, std::max(1, rows - 2));
-    const int ntile = (a.ny + ty - 1) / ty;
+    // y-tiles of ty rows (+ 2 halo rows): up to 1024 nodes, and two LDS planes
+    // within 64 KB; nx <= 64 (attach).  The fewest tiles that fit, with the
+    // rows dealt evenly, so that little is re-read as halo (51^3 box: three
+    // tiles of 17 rows, 57 rows read per 51, where tiles of 8 rows read 70)
+    const int rows = std::min(1024 / a.nx, (int)(65536 / (2 * sizeof(double) * nt * a.nx)));
+    const int cap = std::min(a.ny, std::max(1, rows - 2));
+    const int ntile = (a.ny + cap - 1) / cap;
+    const int ty = (a.ny + ntile - 1) / ntile;
     const int threads = ((ty + 2) * a.nx + 63) / 64 * 64;
     const size_t lds = 2 * sizeof(double) * nt * (ty + 2) * a.nx;
     const long units = (long)ntile * d->range_count();  // the launch range's sites
