@@ -806,117 +806,127 @@ int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps) {
         REQUIRE(!d->box || k_steps % M == 0, CASIM_E_INVALID,
                 "with a cytosol box, k_steps must be a multiple of its substeps");
         d->use();
-        const int cur0 = d->cur, bcur0 = casim::box_cur(d);
-        auto key = std::make_tuple((int)k_steps, dt, cur0 | bcur0 << 1);
-        auto it = d->graphs.find(key);
-        if (it == d->graphs.end()) {
-            HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-            const bool pool_rim = d->Q.on && d->Q.cyto;
-            const bool pairs = d->pair_ok();
-            try {
-                int k = 0;
-                if (d->split && pairs && !pool_rim && d->P.S >= 2 && k_steps >= 2) {
-                    // Split pipelining: the two halves of the batch step on two
-                    // streams; the second starts one channel call after the
-                    // first, so one half's gating runs beside the other half's
-                    // pass.  Sites are independent (a cytosol box too: each
-                    // site's box takes only that site's effluxes), so the
-                    // results are those of the whole batch on one stream.
-                    const int half = (d->P.S + 1) / 2;
-                    const int npairs = k_steps / 2, cur_start = d->cur, bcur_start = casim::box_cur(d);
-                    HIPCK(hipEventRecord(d->ev_fork, d->stream));
-                    HIPCK(hipStreamWaitEvent(d->stream2, d->ev_fork, 0));
-                    for (int h = 0; h < 2; ++h) {
-                        d->range_s0 = h ? half : 0;
-                        d->range_n = h ? d->P.S - half : half;
-                        d->range_stream = h ? d->stream2 : d->stream;
-                        d->cur = cur_start;
-                        casim::box_set_cur(d, bcur_start);
-                        if (h) HIPCK(hipStreamWaitEvent(d->stream2, d->ev_off, 0));
-                        for (int i = 0; i < npairs; ++i) {
-                            const int ks = 2 * i;  // the pair's first step
-                            d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (ks ? CH_ACCUMULATE : 0) |
-                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                               dt);
-                            if (!h && !i) HIPCK(hipEventRecord(d->ev_off, d->stream));
-                            d->launch_pair(dt);
-                            if (d->box) casim::box_efflux_pair(d, ks % M == 0, dt);
-                            d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE |
-                                                   CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                               dt);
-                            if (d->box && (ks + 2) % M == 0) {  // box step on the accumulated effluxes, then the rim
-                                casim::box_step(d, M * dt);
-                                casim::box_rim(d);
+        // One captured graph covers at most 2048 steps, a whole number of
+        // box steps: a longer call replays several graphs, which gives the
+        // results of one (each graph ends with the update that the next
+        // graph's first channel call expects) while the node count of any
+        // one graph, about three per two steps and half-batch, stays bounded.
+        const int unit = 2 * M, cap = std::max(unit, 2048 / unit * unit);
+        for (int32_t left = k_steps; left > 0;) {
+            const int32_t kc = std::min(left, (int32_t)cap);
+            left -= kc;
+            const int cur0 = d->cur, bcur0 = casim::box_cur(d);
+            auto key = std::make_tuple((int)kc, dt, cur0 | bcur0 << 1);
+            auto it = d->graphs.find(key);
+            if (it == d->graphs.end()) {
+                HIPCK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+                const bool pool_rim = d->Q.on && d->Q.cyto;
+                const bool pairs = d->pair_ok();
+                try {
+                    int k = 0;
+                    if (d->split && pairs && !pool_rim && d->P.S >= 2 && kc >= 2) {
+                        // Split pipelining: the two halves of the batch step on two
+                        // streams; the second starts one channel call after the
+                        // first, so one half's gating runs beside the other half's
+                        // pass.  Sites are independent (a cytosol box too: each
+                        // site's box takes only that site's effluxes), so the
+                        // results are those of the whole batch on one stream.
+                        const int half = (d->P.S + 1) / 2;
+                        const int npairs = kc / 2, cur_start = d->cur, bcur_start = casim::box_cur(d);
+                        HIPCK(hipEventRecord(d->ev_fork, d->stream));
+                        HIPCK(hipStreamWaitEvent(d->stream2, d->ev_fork, 0));
+                        for (int h = 0; h < 2; ++h) {
+                            d->range_s0 = h ? half : 0;
+                            d->range_n = h ? d->P.S - half : half;
+                            d->range_stream = h ? d->stream2 : d->stream;
+                            d->cur = cur_start;
+                            casim::box_set_cur(d, bcur_start);
+                            if (h) HIPCK(hipStreamWaitEvent(d->stream2, d->ev_off, 0));
+                            for (int i = 0; i < npairs; ++i) {
+                                const int ks = 2 * i;  // the pair's first step
+                                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (ks ? CH_ACCUMULATE : 0) |
+                                                       CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                                   dt);
+                                if (!h && !i) HIPCK(hipEventRecord(d->ev_off, d->stream));
+                                d->launch_pair(dt);
+                                if (d->box) casim::box_efflux_pair(d, ks % M == 0, dt);
+                                d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE |
+                                                       CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                                   dt);
+                                if (d->box && (ks + 2) % M == 0) {  // box step on the accumulated effluxes, then the rim
+                                    casim::box_step(d, M * dt);
+                                    casim::box_rim(d);
+                                }
                             }
                         }
+                        d->range_s0 = 0;
+                        d->range_n = -1;
+                        d->range_stream = nullptr;
+                        HIPCK(hipEventRecord(d->ev_join, d->stream2));
+                        HIPCK(hipStreamWaitEvent(d->stream, d->ev_join, 0));
+                        k = 2 * npairs;
+                    }
+                    for (; k < kc;) {
+                        if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux
+                            d->launch_rim(d->Q.cions, d->Q.cbb);
+                            d->launch_efflux();
+                        }
+                        const bool pair = pairs && k + 1 < kc;
+                        if (d->box && !pair) casim::box_efflux(d, k % M == 0, dt);
+                        // step 1 recomputes the currents of the preceding update, so
+                        // it does not add to the accumulated totals again
+                        d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
+                                               CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                           dt);
+                        if (pair) {
+                            // steps k and k+1 in one pass, then step k+1's channel
+                            // update from the mouth values the pass stored
+                            d->launch_pair(dt);
+                            if (d->box) casim::box_efflux_pair(d, k % M == 0, dt);  // both steps' rim effluxes
+                            d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
+                                               dt);
+                            k += 2;
+                        } else {
+                            d->launch_step(dt);
+                            k += 1;
+                        }
+                        if (d->box && k % M == 0) {  // box step on the accumulated effluxes, then the rim
+                            casim::box_step(d, M * dt);
+                            casim::box_rim(d);
+                        }
+                    }
+                    if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
+                    d->launch_efflux();
+                    d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, dt);
+                } catch (...) {
+                    // rejoin the second stream (if it was forked) so the capture can end
+                    if (d->range_n >= 0) {
+                        (void)hipEventRecord(d->ev_join, d->stream2);
+                        (void)hipStreamWaitEvent(d->stream, d->ev_join, 0);
                     }
                     d->range_s0 = 0;
                     d->range_n = -1;
                     d->range_stream = nullptr;
-                    HIPCK(hipEventRecord(d->ev_join, d->stream2));
-                    HIPCK(hipStreamWaitEvent(d->stream, d->ev_join, 0));
-                    k = 2 * npairs;
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(d->stream, &junk);
+                    d->cur = cur0;
+                    casim::box_set_cur(d, bcur0);
+                    throw;
                 }
-                for (; k < k_steps;) {
-                    if (pool_rim) {  // the update of the previous step: rim from the cytosol, efflux
-                        d->launch_rim(d->Q.cions, d->Q.cbb);
-                        d->launch_efflux();
-                    }
-                    const bool pair = pairs && k + 1 < k_steps;
-                    if (d->box && !pair) casim::box_efflux(d, k % M == 0, dt);
-                    // step 1 recomputes the currents of the preceding update, so
-                    // it does not add to the accumulated totals again
-                    d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_GATE | (k ? CH_ACCUMULATE : 0) |
-                                           CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                       dt);
-                    if (pair) {
-                        // steps k and k+1 in one pass, then step k+1's channel
-                        // update from the mouth values the pass stored
-                        d->launch_pair(dt);
-                        if (d->box) casim::box_efflux_pair(d, k % M == 0, dt);  // both steps' rim effluxes
-                        d->launch_channels(CH_FROM_STORED | CH_CURRENTS | CH_GATE | CH_ACCUMULATE | CH_POOL_UPDATE | CH_POOL_ADVANCE,
-                                           dt);
-                        k += 2;
-                    } else {
-                        d->launch_step(dt);
-                        k += 1;
-                    }
-                    if (d->box && k % M == 0) {  // box step on the accumulated effluxes, then the rim
-                        casim::box_step(d, M * dt);
-                        casim::box_rim(d);
-                    }
-                }
-                if (pool_rim) d->launch_rim(d->Q.cions, d->Q.cbb);
-                d->launch_efflux();
-                d->launch_channels(CH_SNAPSHOT | CH_CURRENTS | CH_ACCUMULATE | CH_POOL_UPDATE, dt);
-            } catch (...) {
-                // rejoin the second stream (if it was forked) so the capture can end
-                if (d->range_n >= 0) {
-                    (void)hipEventRecord(d->ev_join, d->stream2);
-                    (void)hipStreamWaitEvent(d->stream, d->ev_join, 0);
-                }
-                d->range_s0 = 0;
-                d->range_n = -1;
-                d->range_stream = nullptr;
-                hipGraph_t junk;
-                (void)hipStreamEndCapture(d->stream, &junk);
+                hipGraph_t graph;
+                HIPCK(hipStreamEndCapture(d->stream, &graph));
+                hipGraphExec_t exec;
+                HIPCK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                HIPCK(hipGraphDestroy(graph));
+                const int flips = (d->cur ^ cur0) | (casim::box_cur(d) ^ bcur0) << 1;
                 d->cur = cur0;
                 casim::box_set_cur(d, bcur0);
-                throw;
+                it = d->graphs.emplace(key, std::make_pair(exec, flips)).first;
             }
-            hipGraph_t graph;
-            HIPCK(hipStreamEndCapture(d->stream, &graph));
-            hipGraphExec_t exec;
-            HIPCK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            HIPCK(hipGraphDestroy(graph));
-            const int flips = (d->cur ^ cur0) | (casim::box_cur(d) ^ bcur0) << 1;
-            d->cur = cur0;
-            casim::box_set_cur(d, bcur0);
-            it = d->graphs.emplace(key, std::make_pair(exec, flips)).first;
+            HIPCK(hipGraphLaunch(it->second.first, d->stream));
+            d->cur = cur0 ^ (it->second.second & 1);
+            casim::box_set_cur(d, bcur0 ^ (it->second.second >> 1));
         }
-        HIPCK(hipGraphLaunch(it->second.first, d->stream));
-        d->cur = cur0 ^ (it->second.second & 1);
-        casim::box_set_cur(d, bcur0 ^ (it->second.second >> 1));
     });
 }
 

@@ -488,3 +488,42 @@ def test_gpu_box_split_pipelining_equals_one_stream(gpu, dim):
     assert np.array_equal(one.GetRngStates(), two.GetRngStates())
     assert np.array_equal(one.GetPools()[0], two.GetPools()[0])
     assert np.array_equal(one.GetReleaseStats(), two.GetReleaseStats())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("substeps", [None, 4, 5])
+def test_gpu_long_calls_equal_short_calls(gpu, substeps):
+    """A call of more steps than one captured graph holds (2048, or the most
+    whole box steps below it) replays several graphs, the last one shorter;
+    its results equal those of short calls bit for bit: dyad and box fields,
+    channel and RNG states, jSR and release statistics, with fast gating and
+    a depleting jSR, without a box (two-step passes, split halves, an odd
+    last step) and with boxes of even and odd substeps."""
+    from casim_amd.dyad import DyadHIP
+    say("long calls", substeps)
+    sp, box = small_case(dim=2, n_sites=3, clamped=False, far=True, substeps=substeps or 4)
+    sp.channel_types[0].params["kopen0"] *= 1e4
+    sp.channel_types[0].params["kclose0"] *= 10
+    long_call, short_calls = (4101, [1000] * 4 + [101]) if substeps is None else (4100, [100] * 41)
+    devs = []
+    for calls in ([long_call], short_calls):
+        d = DyadHIP(sp)
+        d.AttachPools(depleting_pool())
+        if substeps:
+            d.AttachCytosolBox(box)
+        d.InitOpeningAll()
+        d.UpdatePools(*([] if substeps else sp.rim_arrays()))
+        for k in calls:
+            d.Step(2e-6, k)
+        devs.append(d)
+    one, many = devs
+    for f in range(sp.n_ions + sp.n_bound):
+        assert np.array_equal(one.GetField(f), many.GetField(f)), f
+    if substeps:
+        for f in range(one.CytosolBoxFields()):
+            assert np.array_equal(one.GetCytosolBox(f), many.GetCytosolBox(f)), f
+    assert np.array_equal(one.GetChannelsStates(), many.GetChannelsStates())
+    assert np.array_equal(one.GetRngStates(), many.GetRngStates())
+    assert np.array_equal(one.GetPools()[0], many.GetPools()[0])
+    assert np.array_equal(one.GetReleaseStats(), many.GetReleaseStats())
+    assert one.GetReleaseStats()[:, 0].max() > 0  # channels opened: the runs are not trivial
