@@ -233,8 +233,10 @@ int casim_dyad_run_mc(CasimDyad *d, double dt, int32_t site);
  * constant jSR (ParallelHybridSolver.cpp:408-437 with DyadRDCPU, per step
  * RunRD + RunMC for every site, then Update).  Equivalent to k_steps times
  * { run_rd(dt); run_mc(dt); update(rim, jsr) } with the rim and jSR of the
- * last casim_dyad_update call.  The k_steps launches are captured once into a
- * HIP graph per (k_steps, dt) and replayed.
+ * last casim_dyad_update call.  The launches are captured once into a HIP
+ * graph per (step count, dt) and replayed; one graph holds at most 2048 steps
+ * (with a cytosol box, the most whole box steps below that), so a longer call
+ * replays several graphs, with the results of one call.
  */
 int casim_dyad_step(CasimDyad *d, double dt, int32_t k_steps);
 
